@@ -1,0 +1,104 @@
+/*
+ * osr_oracle.h -- CPU FP64 restatement of the Osiris per-cell Newton hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP product is
+ * compared against; it is loaded by tests/, by __graft_entry__.smoke() and by
+ * the cpu_baseline leg of bench.py, never by the product path.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference repository root, compsim/...).  Arithmetic is plain IEEE
+ * double, written to follow the reference's operation order where that order
+ * is observable (iteration stopping rules, clamps, special cases).
+ */
+#ifndef OSR_ORACLE_H
+#define OSR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_PR 0
+#define OR_SRK 1
+
+/* Constants the reference compiles in (compsim/Functions.h). */
+#define OR_RGAS 8.3144621             /* Functions.h:216 */
+#define OR_PI 3.1415926535897932384626433
+#define OR_SQRT2 1.4142135623730950488016887
+#define OR_NEWTON_TOL 1e-8            /* Functions.h:236 */
+#define OR_FLASH_TOL 1e-14            /* Functions.h:237 */
+#define OR_DZERO 1e-20                /* Functions.h:239 */
+#define OR_MAX_FLASH_N_ITR 100        /* Functions.h:126 */
+#define OR_MAX_SS_ITER 500            /* Init_Calcs.cpp:714 */
+#define OR_SXMTOL 1e-15               /* Functions.h:251 */
+#define OR_XMTOL 1e-8                 /* Functions.h:224 */
+#define OR_TRIV_K 1e-6                /* Functions.h:269 */
+#define OR_MAX_MNR_ITER 20            /* Functions.h:270 */
+#define OR_PZERO 1000000.0            /* Functions.h:247 */
+#define OR_EPS_SAT 1e-3               /* Functions.h:226 */
+#define OR_PHASE_ENV 0.95             /* Functions.h:11 */
+#define OR_EPS_SAT_FRAC 0.0           /* Functions.h:12 */
+
+typedef struct {
+    int nc;
+    int eos;              /* OR_PR or OR_SRK */
+    const double *Tc;     /* [nc] K   */
+    const double *Pc;     /* [nc] Pa  */
+    const double *w;      /* [nc] acentric factor */
+    const double *bic;    /* [nc*nc] row-major, holds 1 - k_ij (Data_Input.cpp:977-990) */
+} or_fluid;
+
+/* EOS_Init (Init_Calcs.cpp:66-97): a_i(T), b_i for PR or SRK. */
+void or_eos_ab(const or_fluid *f, double T, double *a, double *b);
+
+/* Solve_Z (Globals.cpp:212-264): root of Z^3 + a1 Z^2 + a2 Z + a3, gas=1 picks
+ * the vapour root, gas=0 the liquid root. */
+double or_solve_z(double a1, double a2, double a3, int gas);
+
+/* One phase: mixing, Z root and ln(phi_i) (canonical form Init_Calcs.cpp:633-712).
+ * x need not be normalised (the reference mixes unnormalised compositions). */
+void or_phase(const or_fluid *f, const double *a, const double *b, double T,
+              double P, const double *x, int gas, double *Z, double *lnphi);
+
+typedef struct {
+    double L, Zl, Zg, so, sg, rho_l, rho_g;
+    int label;   /* 1 liquid, 0 two-phase, -1 gas (Functions.h:327) */
+    int iters;   /* outer successive-substitution iterations */
+} or_flash_out;
+
+/* SuccFalsh (Init_Calcs.cpp:571-738) for one cell. */
+void or_flash(const or_fluid *f, double T, double P, const double *z, double sw,
+              double *x, double *y, double *K, or_flash_out *o);
+
+/* Batched SuccFalsh over n cells, structure-of-arrays ([comp][cell]).
+ * T: per-cell temperatures or NULL (then T0 is used).  Outputs may be NULL. */
+void or_flash_batch(const or_fluid *f, int64_t n, const double *T, double T0,
+                    const double *P, const double *z, double sw,
+                    double *x, double *y, double *K, double *L, int8_t *label,
+                    double *Zl, double *Zg, int32_t *iters, int nthreads);
+
+/* Bubl_Succ (Minor_NR.cpp:491-871).  x: liquid composition (comp[..][0]),
+ * ystale: the cell's stored vapour composition (comp[..][1]), which the
+ * reference's successive-substitution stage reads.  Returns Psat or -1;
+ * xm[0..nc-1] receives the incipient composition left in the global Xm. */
+double or_bubble(const or_fluid *f, double T, double P, const double *x,
+                 const double *ystale, double *xm);
+
+/* Dew_Succ (Minor_NR.cpp:873-1249).  y: vapour composition. */
+double or_dew(const or_fluid *f, double T, double P, const double *y, double *xm);
+
+/* Batched saturation pressure: kind[c] = 1 bubble, -1 dew, 0 skip (psat = -1). */
+void or_psat_batch(const or_fluid *f, int64_t n, const double *T, double T0,
+                   const double *P, const int8_t *kind, const double *x,
+                   const double *y, double *psat, double *xm, int nthreads);
+
+/* Dense LU with partial pivoting + solve, the arithmetic LAPACK dgesv performs
+ * (called through MKLS, MKL_Solver.Cpp:66-90).  A is n*n row-major and is
+ * overwritten.  Returns 0, or k>0 if U(k-1,k-1) is exactly zero. */
+int or_lu_solve(int n, double *A, const double *rhs, double *sol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

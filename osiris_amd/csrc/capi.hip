@@ -1,0 +1,106 @@
+// capi.hip -- the extern "C" entry points of libosiris_amd.so.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "osr_internal.hpp"
+
+namespace {
+
+// Temperature-independent EOS constants (EOS_Init, compsim/Init_Calcs.cpp:66-97):
+// ac_i and m_i are kept apart so each cell can form a_i(T) for its own T.
+template <class R>
+void fill_consts(osr::FluidConsts<R>& C, const osr_fluid* f) {
+    const int nc = f->nc;
+    const bool srk = f->eos == OSR_EOS_SRK;
+    const double R2 = osr::RGAS * osr::RGAS;
+    for (int i = 0; i < nc; i++) {
+        const double Tc = f->Tc[i], Pc = f->Pc[i], w = f->omega[i];
+        double ac, m, b;
+        if (srk) {
+            ac = 0.42747 * R2 * Tc * Tc / Pc;
+            m = 0.48508 + 1.55171 * w - 0.15613 * w * w;
+            b = 0.08664 * osr::RGAS * Tc / Pc;
+        } else {
+            ac = 0.457235 * R2 * Tc * Tc / Pc;
+            m = 0.3796 + 1.485 * w - 0.1644 * w * w + 0.01667 * w * w * w;
+            b = 0.077796 * osr::RGAS * Tc / Pc;
+        }
+        C.Tc[i] = R(Tc);
+        C.Pc[i] = R(Pc);
+        C.sac[i] = R(std::sqrt(ac));
+        C.m[i] = R(m);
+        C.b[i] = R(b);
+        C.wk[i] = R(5.37 * (1 + w));
+        for (int j = 0; j < nc; j++) C.bic[i * nc + j] = R(1.0 - f->kij[i * nc + j]);
+    }
+    C.d1 = R(srk ? 1.0 : 1.0 + osr::SQRT2);
+    C.d2 = R(srk ? 0.0 : 1.0 - osr::SQRT2);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* osr_version(void) { return "osiris_amd 0.1 gfx950"; }
+
+int osr_supported_nc(int32_t nc) { return osr::nc_supported(nc) ? 1 : 0; }
+
+int osr_ctx_create(const osr_fluid* fluid, osr_ctx** out) {
+    if (!fluid || !out || !fluid->Tc || !fluid->Pc || !fluid->omega || !fluid->kij) return OSR_E_ARG;
+    if (fluid->nc < 1 || fluid->nc > osr::NCMAX) return OSR_E_ARG;
+    if (fluid->eos != OSR_EOS_PR && fluid->eos != OSR_EOS_SRK) return OSR_E_ARG;
+    if (!osr::nc_supported(fluid->nc)) return OSR_E_NC;
+    for (int i = 0; i < fluid->nc; i++)
+        if (!(fluid->Tc[i] > 0) || !(fluid->Pc[i] > 0)) return OSR_E_ARG;
+    osr_ctx* c = new (std::nothrow) osr_ctx();
+    if (!c) return OSR_E_ARG;
+    c->nc = fluid->nc;
+    c->eos = fluid->eos;
+    c->host_fluid.nc = fluid->nc;
+    c->host_fluid.eos = fluid->eos;
+    fill_consts<double>(c->host_fluid.d, fluid);
+    fill_consts<float>(c->host_fluid.f, fluid);
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
+        delete c;
+        return OSR_E_HIP;
+    }
+    c->num_cu = prop.multiProcessorCount;
+    if (hipMalloc(&c->d_fluid, sizeof(osr::FluidDev)) != hipSuccess ||
+        hipMalloc(&c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemcpy(c->d_fluid, &c->host_fluid, sizeof(osr::FluidDev), hipMemcpyHostToDevice) != hipSuccess) {
+        osr_ctx_destroy(c);
+        return OSR_E_HIP;
+    }
+    *out = c;
+    return OSR_OK;
+}
+
+int osr_ctx_destroy(osr_ctx* ctx) {
+    if (!ctx) return OSR_E_ARG;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipSetDevice(ctx->device);
+    if (ctx->d_fluid) hipFree(ctx->d_fluid);
+    if (ctx->d_counters) hipFree(ctx->d_counters);
+    if (ctx->d_queue) hipFree(ctx->d_queue);
+    hipSetDevice(dev);
+    delete ctx;
+    return OSR_OK;
+}
+
+int osr_flash_device(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
+                     const double* z, double sw, const osr_flash_out* out, int mode,
+                     void* stream) {
+    return osr::launch_flash(ctx, n, T, T0, P, z, sw, out, mode, (hipStream_t)stream);
+}
+
+int osr_psat_device(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
+                    const int8_t* kind, const double* x, const double* y, double* psat,
+                    double* xm, void* stream) {
+    return osr::launch_psat(ctx, n, T, T0, P, kind, x, y, psat, xm, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,46 @@
+// osr_internal.hpp -- host-side context and helpers shared by the HIP
+// translation units of libosiris_amd.so.  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/osiris_amd.h"
+#include "eos_dev.hpp"
+
+struct osr_ctx {
+    int device = 0;
+    int nc = 0;
+    int eos = 0;
+    int num_cu = 0;
+    osr::FluidDev host_fluid{};
+    osr::FluidDev* d_fluid = nullptr;
+    // Work-queue heads, one per queue used inside a call (zeroed by a fill
+    // kernel on the call's stream, so the call stays graph-capturable).
+    unsigned long long* d_counters = nullptr;
+    // Compaction queue of unconverged flash cells (grown on demand).
+    void* d_queue = nullptr;
+    size_t queue_bytes = 0;
+    // Tuning knobs (set by osr_tune_set; defaults chosen from measurement).
+    int flash_block = 256;
+    int flash_waves_per_simd = 0;   // 0 = from occupancy query
+    int refill_min = 16;
+    int mixed_warm_iters = 2;
+    float mixed_tol32 = 1e-5f;
+    int mixed_max32 = 200;
+};
+
+#define OSR_HIP(call)                                   \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return OSR_E_HIP;         \
+    } while (0)
+
+namespace osr {
+int launch_flash(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
+                 const double* z, double sw, const osr_flash_out* out, int mode,
+                 hipStream_t stream);
+int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
+                const int8_t* kind, const double* x, const double* y, double* psat,
+                double* xm, hipStream_t stream);
+bool nc_supported(int nc);
+}  // namespace osr

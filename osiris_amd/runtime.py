@@ -1,0 +1,177 @@
+"""ctypes binding of libosiris_amd.so and torch-tensor front ends.
+
+Mirrors the reference's per-cell entry points by role: ``flash`` is
+SuccFalsh/AllFlash (compsim/Init_Calcs.cpp:571-738, 949-962), ``psat`` is
+Bubl_Succ/Dew_Succ (compsim/Minor_NR.cpp:491-1249).  All arrays are torch
+tensors on the current HIP device, float64, structure-of-arrays
+``[component, cell]``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libosiris_amd.so")
+
+FLASH_PLAIN, FLASH_PERSISTENT, FLASH_MIXED = 0, 1, 2
+_MODES = {"plain": FLASH_PLAIN, "persistent": FLASH_PERSISTENT, "mixed": FLASH_MIXED}
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+class OsrError(RuntimeError):
+    pass
+
+
+class _Fluid(C.Structure):
+    _fields_ = [("nc", C.c_int32), ("eos", C.c_int32), ("Tc", _dp), ("Pc", _dp),
+                ("omega", _dp), ("kij", _dp)]
+
+
+class _FlashOut(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in
+                ("x", "y", "K", "L", "label", "Zl", "Zg", "so", "sg", "rho_l", "rho_g",
+                 "iters", "rr_iters")]
+
+
+def lib():
+    """Load libosiris_amd.so; raise loudly if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OsrError(f"{LIB_PATH} is missing: run osiris_amd/build.py (or "
+                           "__graft_entry__.build()) first; there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        L.osr_version.restype = C.c_char_p
+        L.osr_ctx_create.argtypes = [C.POINTER(_Fluid), C.POINTER(C.c_void_p)]
+        L.osr_ctx_destroy.argtypes = [C.c_void_p]
+        L.osr_flash_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_double, C.c_void_p,
+                                       C.c_void_p, C.c_double, C.POINTER(_FlashOut), C.c_int,
+                                       C.c_void_p]
+        L.osr_psat_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_double, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise OsrError(f"{what} failed with status {rc}")
+    return rc
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class Context:
+    """Owns an osr_ctx (fluid constants uploaded to the current device)."""
+
+    def __init__(self, fluid):
+        self.fluid = fluid
+        self.nc = int(fluid.nc)
+        self._arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (fluid.Tc, fluid.Pc, fluid.omega, np.asarray(fluid.kij).reshape(-1))]
+        s = _Fluid(self.nc, int(fluid.eos), *[a.ctypes.data_as(_dp) for a in self._arrs])
+        h = C.c_void_p()
+        _check(lib().osr_ctx_create(C.byref(s), C.byref(h)), "osr_ctx_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().osr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _as_dev(t, dtype=None):
+    import torch
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(np.asarray(t))
+    t = t.to(device="cuda", dtype=dtype or t.dtype)
+    return t.contiguous()
+
+
+def alloc_flash_out(nc, n, device="cuda", full=True):
+    import torch
+    f64 = dict(dtype=torch.float64, device=device)
+    out = dict(x=torch.empty(nc, n, **f64), y=torch.empty(nc, n, **f64),
+               K=torch.empty(nc, n, **f64), L=torch.empty(n, **f64),
+               label=torch.empty(n, dtype=torch.int8, device=device),
+               Zl=torch.empty(n, **f64), Zg=torch.empty(n, **f64),
+               iters=torch.empty(n, dtype=torch.int32, device=device))
+    if full:
+        out.update(so=torch.empty(n, **f64), sg=torch.empty(n, **f64),
+                   rho_l=torch.empty(n, **f64), rho_g=torch.empty(n, **f64),
+                   rr_iters=torch.empty(n, dtype=torch.int32, device=device))
+    return out
+
+
+def flash(ctx, T, P, z, sw=0.0, mode="persistent", out=None, stream=None):
+    """Flash n cells (SuccFalsh).  T: scalar or [n]; P: [n]; z: [nc, n]."""
+    import torch
+    z = _as_dev(z, torch.float64)
+    nc, n = z.shape
+    if nc != ctx.nc:
+        raise ValueError(f"z has {nc} components, context has {ctx.nc}")
+    P = _as_dev(P, torch.float64)
+    if P.shape != (n,):
+        raise ValueError("P must be [n]")
+    if np.isscalar(T):
+        Tt, T0 = None, float(T)
+    else:
+        Tt, T0 = _as_dev(T, torch.float64), 0.0
+        if Tt.shape != (n,):
+            raise ValueError("T must be scalar or [n]")
+    if out is None:
+        out = alloc_flash_out(nc, n)
+    fo = _FlashOut(*[(_ptr(out.get(k)).value if out.get(k) is not None else None)
+                     for k, _ in _FlashOut._fields_])
+    rc = lib().osr_flash_device(ctx.h, n, _ptr(Tt), T0, _ptr(P), _ptr(z), float(sw),
+                                C.byref(fo), _MODES[mode] if isinstance(mode, str) else int(mode),
+                                _stream_ptr(stream))
+    _check(rc, "osr_flash_device")
+    return out
+
+
+def psat(ctx, T, P, kind, x, y, stream=None):
+    """Saturation pressure (kind 1 bubble of x, -1 dew of y).  Returns (psat, xm)."""
+    import torch
+    x = _as_dev(x, torch.float64)
+    y = _as_dev(y, torch.float64)
+    nc, n = x.shape
+    P = _as_dev(P, torch.float64)
+    kind = _as_dev(kind, torch.int8)
+    if np.isscalar(T):
+        Tt, T0 = None, float(T)
+    else:
+        Tt, T0 = _as_dev(T, torch.float64), 0.0
+    ps = torch.empty(n, dtype=torch.float64, device="cuda")
+    xm = torch.empty(nc, n, dtype=torch.float64, device="cuda")
+    _check(lib().osr_psat_device(ctx.h, n, _ptr(Tt), T0, _ptr(P), _ptr(kind), _ptr(x), _ptr(y),
+                                 _ptr(ps), _ptr(xm), _stream_ptr(stream)), "osr_psat_device")
+    return ps, xm
+
+
+def psat_numpy(ctx):
+    """psat front end with numpy in/out (for the synthetic-input generator)."""
+    def fn(T, P, kind, x, y):
+        ps, xm = psat(ctx, T, P, kind, x, y)
+        return ps.cpu().numpy(), xm.cpu().numpy()
+    return fn
