@@ -151,3 +151,129 @@ def lu_solve(A, b):
     x = np.zeros(n)
     info = lib().or_lu_solve(n, _p(A), _p(b), _p(x))
     return x, info
+
+
+# ------------------------------------------------------------------ assembly
+class _FluidExt(C.Structure):
+    _fields_ = [("MW", _dp), ("Vc", _dp), ("Zc", _dp), ("parachor", _dp)]
+
+
+class _Grid(C.Structure):
+    _fields_ = [("dx", _dp), ("dy", _dp), ("dz", _dp), ("phi_m", _dp), ("phi_f", _dp),
+                ("perm_m", _dp), ("perm_f", _dp), ("tor", _dp), ("swt", _dp), ("sgt", _dp),
+                ("refP", C.c_double), ("cpor", C.c_double), ("dcpor", C.c_double),
+                ("sigma", C.c_double), ("lf", C.c_double), ("cdiff", C.c_double),
+                ("T", C.c_double), ("dt", C.c_double), ("wat_ro", C.c_double),
+                ("wat_mu", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("const_diff", C.c_int32), ("nswt", C.c_int32), ("nsgt", C.c_int32)]
+
+
+_i8p = C.POINTER(C.c_int8)
+
+
+class _State(C.Structure):
+    _fields_ = [("label_m", _i8p), ("label_f", _i8p), ("Po_m", _dp), ("Po_f", _dp),
+                ("x_m", _dp), ("y_m", _dp), ("x_f", _dp), ("y_f", _dp), ("S_m", _dp),
+                ("S_f", _dp), ("rho_prev_m", _dp), ("pre_m", _dp), ("pre_f", _dp),
+                ("pcoeff", C.c_double)]
+
+
+class _Asm:
+    """Keeps the numpy buffers alive behind the C structs of one case."""
+
+    def __init__(self, fluid, grid, state):
+        self.fluid = OracleFluid(fluid)
+        g = grid
+        self.keep = []
+
+        def P(a, dt=np.float64):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            self.keep.append(a)
+            return a.ctypes.data_as(_dp if dt == np.float64 else _i8p)
+
+        self.ext = _FluidExt(P(fluid.MW), P(fluid.Vc), P(fluid.Zc), P(fluid.parachor))
+        self.grid = _Grid(P(g.dx), P(g.dy), P(g.dz), P(g.phi_m), P(g.phi_f), P(g.perm_m),
+                          P(g.perm_f), P(g.tor), P(g.swt), P(g.sgt), g.refP, g.cpor, g.dcpor,
+                          g.sigma, g.lf, g.cdiff, g.T, g.dt, g.wat_ro, g.wat_mu, g.nx, g.ny,
+                          g.nz, int(g.const_diff), len(g.swt), len(g.sgt))
+        s = state
+        self.state = _State(P(s.label_m, np.int8), P(s.label_f, np.int8), P(s.Po_m), P(s.Po_f),
+                            P(s.x_m), P(s.y_m), P(s.x_f), P(s.y_f), P(s.S_m), P(s.S_f),
+                            P(s.rho_prev_m), P(getattr(s, "pre_m", None)),
+                            P(getattr(s, "pre_f", None)), s.pcoeff)
+        self.n = g.nx * g.ny * g.nz
+        self.nc = fluid.nc
+
+
+def layout(fluid, grid, state):
+    a = _Asm(fluid, grid, state)
+    bs = np.zeros(2 * a.n, np.int32)
+    off = np.zeros(2 * a.n + 1, np.int64)
+    lib().or_layout(C.byref(a.grid), C.byref(a.state), a.nc, bs.ctypes.data_as(C.c_void_p),
+                    off.ctypes.data_as(C.c_void_p))
+    return bs, off
+
+
+def residual(fluid, grid, state):
+    a = _Asm(fluid, grid, state)
+    _, off = layout(fluid, grid, state)
+    F = np.zeros(int(off[-1]))
+    lib().or_residual(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state), _p(F))
+    return F
+
+
+def props(fluid, grid, state):
+    a = _Asm(fluid, grid, state)
+    w = 16 + 2 * a.nc
+    out = np.zeros((2, a.n, w))
+    lib().or_props(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state), _p(out))
+    return out
+
+
+def jacobian(fluid, grid, state):
+    """Reference-layout Jacobian as (rowptr, cols, vals)."""
+    a = _Asm(fluid, grid, state)
+    _, off = layout(fluid, grid, state)
+    nrow = int(off[-1])
+    rowptr = np.zeros(nrow + 1, np.int64)
+    cap = nrow * (8 * (2 * a.nc + 4))
+    cols = np.zeros(cap, np.int32)
+    vals = np.zeros(cap)
+    L = lib()
+    L.or_jacobian.restype = C.c_int64
+    nnz = L.or_jacobian(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state),
+                        rowptr.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+                        _p(vals), C.c_int64(cap))
+    assert nnz >= 0, "Jacobian capacity too small"
+    return rowptr, cols[:nnz].copy(), vals[:nnz].copy()
+
+
+def pack(fluid, grid, state):
+    a = _Asm(fluid, grid, state)
+    _, off = layout(fluid, grid, state)
+    X = np.zeros(int(off[-1]))
+    lib().or_pack(C.byref(a.grid), C.byref(a.state), a.nc, _p(X))
+    return X
+
+
+def unpack(fluid, grid, state, X):
+    """UpdateAll: returns a new state object with arrays updated from X."""
+    import copy
+    s2 = copy.copy(state)
+    for k in ("Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f", "S_m", "S_f"):
+        setattr(s2, k, np.array(getattr(state, k), dtype=np.float64, copy=True))
+    a = _Asm(fluid, grid, s2)
+    X = _c(X)
+    # the mutable view aliases the same buffers the _Asm holds
+    lib().or_unpack(C.byref(a.grid), C.byref(a.state), a.nc, _p(X))
+    names = ("Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f", "S_m", "S_f")
+    ptrs = {k: getattr(a.state, k) for k in names}
+    n = a.n
+    sizes = {"Po_m": n, "Po_f": n, "x_m": a.nc * n, "y_m": a.nc * n, "x_f": a.nc * n,
+             "y_f": a.nc * n, "S_m": 3 * n, "S_f": 3 * n}
+    for k in names:
+        arr = np.ctypeslib.as_array(ptrs[k], shape=(sizes[k],)).copy()
+        setattr(s2, k, arr.reshape(np.shape(getattr(state, k))))
+    return s2

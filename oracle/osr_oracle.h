@@ -98,6 +98,58 @@ void or_psat_batch(const or_fluid *f, int64_t n, const double *T, double T0,
  * overwritten.  Returns 0, or k>0 if U(k-1,k-1) is exactly zero. */
 int or_lu_solve(int n, double *A, const double *rhs, double *sol);
 
+
+/* ------------------------------------------------------------ assembly */
+/* Component data beyond the flash (Data_Input.cpp:657-951 library fields). */
+typedef struct {
+    const double *MW;        /* [nc] g/mol */
+    const double *Vc;        /* [nc] cm^3/mol (the deck unit; EOS_Init divides by 1e6) */
+    const double *Zc;        /* [nc] */
+    const double *parachor;  /* [nc] */
+} or_fluid_ext;
+
+/* Grid, rock and run constants (deck GRID, DI/DJ/DK, POR/FPOR, PERM*, TOR,
+ * REFP/CPOR/DCPOR, SIGMA, Lf, CDIF, RESTEMP, DT, WATERPROPS, SWT/SGT). */
+typedef struct {
+    const double *dx, *dy, *dz;          /* [nx], [ny], [nz] m */
+    const double *phi_m, *phi_f;         /* [n] porosity */
+    const double *perm_m, *perm_f;       /* [3*n] (i, j, k) mD */
+    const double *tor;                   /* [n] tortuosity, NULL = 1 */
+    const double *swt, *sgt;             /* [rows*4]: SAT, KR, KRO, PC */
+    double refP, cpor, dcpor, sigma, lf, cdiff, T, dt, wat_ro, wat_mu;
+    int32_t nx, ny, nz, const_diff, nswt, nsgt;
+} or_grid;
+
+/* Per-cell state of both continua (n = nx*ny*nz, cell = iz*nx*ny+iy*nx+ix). */
+typedef struct {
+    const int8_t *label_m, *label_f;     /* [n] 1 L, 0 2ph, -1 G */
+    const double *Po_m, *Po_f;           /* [n] oil pressure, Pa */
+    const double *x_m, *y_m, *x_f, *y_f; /* [nc*n] */
+    const double *S_m, *S_f;             /* [3*n] Sw, So, Sg */
+    const double *rho_prev_m;            /* [2*n] matrix oil/gas density of the
+                                            previous sweep (read by CalcIFT), or NULL */
+    const double *pre_m, *pre_f;         /* [(nc+1)*n] preProp / fpreProp (Pre_Mul), or
+                                            NULL = computed from this state (then the
+                                            accumulation terms vanish, as at X0) */
+    double pcoeff;                       /* PCoeff (CalcPCoeff) */
+} or_state;
+
+typedef struct {
+    const int8_t *label_m, *label_f;
+    double *Po_m, *Po_f, *x_m, *y_m, *x_f, *y_f, *S_m, *S_f;
+    const double *rho_prev_m;
+    const double *pre_m, *pre_f;
+    double pcoeff;
+} or_state_mut;
+
+int or_layout(const or_grid *g, const or_state *s, int nc, int32_t *bsize, int64_t *offset);
+int or_residual(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *F);
+int or_props(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *out);
+int64_t or_jacobian(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s,
+                    int64_t *rowptr, int32_t *cols, double *vals, int64_t cap);
+int or_unpack(const or_grid *g, or_state_mut *s, int nc, const double *X);
+int or_pack(const or_grid *g, const or_state *s, int nc, double *X);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,3 +103,100 @@ def flash_cells(fluid, n, seed, psat_fn, alpha=1.5, T_range=(330.0, 420.0),
     ok = ps > 0
     P = np.where(ok, ps * u, pf)
     return FlashCells(T=T, P=P, z=z, psat=np.where(ok, ps, -1.0))
+
+
+# --------------------------------------------------------------- grids
+# Saturation tables of the reference's example deck (compsim/test.dat:26-55),
+# columns SAT, KR, KRO, PC (Pa).
+SGT_TEST_DAT = np.array([
+    [0, 0, 1, 0], [0.031, 0.0131, 0.6702, 40667.85714], [0.063, 0.0173, 0.55, 51765.35714],
+    [0.094, 0.0226, 0.4395, 62035.71429], [0.125, 0.0297, 0.3395, 72030.35714],
+    [0.188, 0.0515, 0.1721, 94087.5], [0.298, 0.1399, 0.028, 256483.2143],
+    [0.386, 0.3939, 0.002, 419085.7143], [0.438, 0.5456, 0, 1671517.857],
+    [0.565, 0.6215, 0, 2566900], [0.684, 0.6784, 0, 3995100], [0.8, 1, 0, 26824242.86]])
+SWT_TEST_DAT = np.array([
+    [0.2, 0, 1, 1349897.143], [0.229, 0.0001, 0.7407, 656200], [0.255, 0.0003, 0.6829, 413571.4286],
+    [0.308, 0.0012, 0.5722, 151642.8571], [0.334, 0.0023, 0.5194, 129585.7143],
+    [0.412, 0.0102, 0.3715, 86850], [0.464, 0.0219, 0.1526, 58106.78571],
+    [0.557, 0.0416, 0.0822, 30328.57143], [0.606, 0.0721, 0, 9098.571429],
+    [0.647, 0.1448, 0, 0], [0.7, 0.178, 0, 0], [0.8, 0.2604, 0, 0], [1, 1, 0, 0]])
+
+
+@dataclass
+class Grid:
+    nx: int
+    ny: int
+    nz: int
+    dx: np.ndarray
+    dy: np.ndarray
+    dz: np.ndarray
+    phi_m: np.ndarray
+    phi_f: np.ndarray
+    perm_m: np.ndarray      # [3, n] mD
+    perm_f: np.ndarray
+    tor: np.ndarray
+    swt: np.ndarray
+    sgt: np.ndarray
+    refP: float = 101325.0
+    cpor: float = 0.0
+    dcpor: float = 0.0
+    sigma: float = 1.0 / 3.0
+    lf: float = 0.01
+    const_diff: bool = False
+    cdiff: float = 0.0
+    T: float = 360.0
+    dt: float = 86400.0
+    wat_ro: float = 1000.0
+    wat_mu: float = 1.0
+
+    @property
+    def n(self):
+        return self.nx * self.ny * self.nz
+
+
+@dataclass
+class State:
+    label_m: np.ndarray      # [n] int8
+    label_f: np.ndarray
+    Po_m: np.ndarray         # [n]
+    Po_f: np.ndarray
+    x_m: np.ndarray          # [nc, n]
+    y_m: np.ndarray
+    x_f: np.ndarray
+    y_f: np.ndarray
+    S_m: np.ndarray          # [3, n]
+    S_f: np.ndarray
+    rho_prev_m: np.ndarray = None   # [2, n] or None
+    pcoeff: float = 1.0
+    pre_m: np.ndarray = None        # [nc+1, n] or None
+    pre_f: np.ndarray = None
+
+
+def dual_porosity_case(fluid, nx, ny, nz, seed, flash_fn, dx=10.0, dy=10.0, dz=5.0,
+                       p_range=(5e6, 25e6), alpha=2.0, T=360.0, sw_m=0.2, sw_f=0.0,
+                       phi_m=0.2, phi_f=0.01, k_m=0.1, k_f=100.0, k_sigma=0.5, **grid_kw):
+    """A dual-porosity grid and its initial state.  Per cell and continuum
+    p ~ U(p_range), z ~ Dirichlet(alpha); the state is the flash of (p, z)
+    (labels, x, y and the So/Sg split of 1 - Sw), as the reference's
+    initialisation produces it (AllFlash, Init_Calcs.cpp:949-962).
+    flash_fn(T, P, z, sw) -> dict(label, x, y, so, sg)."""
+    g = rng(seed)
+    n = nx * ny * nz
+    grid = Grid(nx, ny, nz, np.full(nx, dx), np.full(ny, dy), np.full(nz, dz),
+                np.full(n, phi_m), np.full(n, phi_f),
+                np.repeat(np.exp(np.log(k_m) + k_sigma * g.normal(size=n))[None], 3, 0).copy(),
+                np.repeat(np.exp(np.log(k_f) + k_sigma * g.normal(size=n))[None], 3, 0).copy(),
+                np.ones(n), SWT_TEST_DAT.copy(), SGT_TEST_DAT.copy(), T=T, **grid_kw)
+    st = {}
+    for cont, sw in (("m", sw_m), ("f", sw_f)):
+        P = g.uniform(p_range[0], p_range[1], n)
+        z = g.dirichlet(np.full(fluid.nc, alpha), n).T.copy()
+        r = flash_fn(T, P, z, sw)
+        st[cont] = (P, r, sw)
+    def S(r, sw):
+        return np.stack([np.full(n, sw), r["so"], r["sg"]])
+    (Pm, rm, swm), (Pf, rf, swf) = st["m"], st["f"]
+    state = State(rm["label"].astype(np.int8), rf["label"].astype(np.int8), Pm, Pf,
+                  rm["x"], rm["y"], rf["x"], rf["y"], S(rm, swm), S(rf, swf),
+                  pcoeff=float(max(Pm.max(), Pf.max())))
+    return grid, state

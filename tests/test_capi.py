@@ -1,0 +1,50 @@
+"""CPU checks of the C ABI: the library loads, exports every entry point the
+headers declare, and rejects bad arguments before touching a device."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+
+from osiris_amd import runtime as rt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names.update(re.findall(r"\b(osr_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    lib = rt.lib()
+    names = declared_symbols()
+    assert len(names) >= 6
+    missing = [s for s in names if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_version_and_supported_nc():
+    lib = rt.lib()
+    assert b"gfx950" in lib.osr_version()
+    for nc in (2, 3, 4, 5, 6, 8, 10, 12):
+        assert lib.osr_supported_nc(nc) == 1
+    for nc in (0, 1, 7, 13, 17):
+        assert lib.osr_supported_nc(nc) == 0
+
+
+def test_ctx_create_rejects_bad_arguments():
+    lib = rt.lib()
+    h = C.c_void_p()
+    assert lib.osr_ctx_create(None, C.byref(h)) == -1
+    arrs = [np.ones(7) * 300, np.ones(7) * 4e6, np.zeros(7), np.zeros(49)]
+    ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
+    assert lib.osr_ctx_create(C.byref(rt._Fluid(0, 0, *ptrs)), C.byref(h)) == -1
+    assert lib.osr_ctx_create(C.byref(rt._Fluid(7, 0, *ptrs)), C.byref(h)) == -3
+    assert lib.osr_ctx_create(C.byref(rt._Fluid(7, 5, *ptrs)), C.byref(h)) == -1
+    assert lib.osr_flash_device(None, 1, None, 300.0, None, None, 0.0, None, 1, None) == -1
