@@ -93,6 +93,118 @@ int osr_psat_device(osr_ctx *ctx, int64_t n, const double *T, double T0,
                     const double *P, const int8_t *kind, const double *x,
                     const double *y, double *psat, double *xm, void *stream);
 
+/* Phase-state control of n cells of one continuum after an accepted Newton
+ * update: PhaseCtrl / fPhaseCtrl (Do_Cycle.cpp:1558-1643 / 1644-1722).  Gas
+ * cells with Pg <= 0.95 x dew pressure and liquid cells with Po <= 0.95 x
+ * bubble pressure become two-phase (incipient saturation 1e-3, incipient
+ * composition from the saturation routine); other cells with So <= 0 (Sg <= 0)
+ * become gas (liquid); saturations are renormalised.  label [n], x, y [nc*n]
+ * and S [3*n] (Sw, So, Sg) are updated in place; Po, Pg [n] are the oil and
+ * gas phase pressures; psat [n] (may be NULL) receives the saturation
+ * pressure evaluated (-1 where none).  Uses a library-owned workspace. */
+int osr_phase_ctrl_device(osr_ctx *ctx, int64_t n, const double *T, double T0, int8_t *label,
+                          const double *Po, const double *Pg, double *x, double *y, double *S,
+                          double *psat, void *stream);
+
+/* ================================================================ assembly
+ * Residual and Jacobian of the dual-porosity model in the reference's layout:
+ * all matrix blocks then all fracture blocks, cell index iz*nx*ny+iy*nx+ix,
+ * block size nc+3 (single-phase) or 2nc+4 (two-phase) (PJac,
+ * Init_Calcs.cpp:1013-1116; MatSetVariableBlockSizes, PetscAUX.cpp:160-163).
+ * Unknowns per block in InitGuess2 order (PetscAUX.cpp:892-957): liquid
+ * [x(nc), P/PCoeff, Sw, So]; gas [y(nc), P/PCoeff, Sw, Sg]; two-phase
+ * [x(nc), y(nc), P/PCoeff, Sw, So, Sg].  Rows in IteratorFunction order
+ * (Petsc_Functions.cpp:3446-6870): two-phase 0..nc-1 fugacity, nc..2nc-2
+ * component balances 0..nc-2, 2nc-1 total hydrocarbon, 2nc water, then
+ * sum x - 1, sum y - 1, sum S - 1; single-phase 0..nc-2 components, nc-1
+ * total, nc water, sum x|y - 1, sum S - 1. */
+
+/* Component data beyond the flash (library fields of Data_Input.cpp:657-951). */
+typedef struct osr_fluid_ext {
+    const double *MW;        /* [nc] g/mol */
+    const double *Vc;        /* [nc] cm^3/mol (deck unit; EOS_Init divides by 1e6) */
+    const double *Zc;        /* [nc] */
+    const double *parachor;  /* [nc] */
+} osr_fluid_ext;
+
+/* Grid, rock and run constants (host memory; deck GRID, DI/DJ/DK, POR/FPOR,
+ * PERMI..K/FPERMI..K, TOR, REFP/CPOR/DCPOR, SIGMA, Lf, CDIF, RESTEMP, DT,
+ * WATERPROPS, SWT/SGT -- Data_Input.cpp:13-1349). */
+typedef struct osr_grid {
+    const double *dx, *dy, *dz;          /* [nx], [ny], [nz] m */
+    const double *phi_m, *phi_f;         /* [n] porosity (matrix, fracture) */
+    const double *perm_m, *perm_f;       /* [3*n] (i, j, k) mD */
+    const double *tor;                   /* [n] tortuosity, NULL = 1 */
+    const double *swt, *sgt;             /* [rows*4] SAT, KR, KRO, PC(Pa); <= 32 rows */
+    double refP, cpor, dcpor;            /* rock compressibility */
+    double sigma;                        /* SIGMA (shape-factor multiplier) */
+    double lf;                           /* Lf, half fracture opening, m */
+    double cdiff;                        /* CDIF value */
+    double T;                            /* reservoir temperature, K */
+    double dt;                           /* time step, s */
+    double wat_ro, wat_mu;               /* WATERPROPS */
+    int32_t nx, ny, nz;
+    int32_t const_diff;                  /* 1 when the deck has CDIF */
+    int32_t nswt, nsgt;
+} osr_grid;
+
+/* Cell state of both continua (device pointers; n = nx*ny*nz). */
+typedef struct osr_state_dev {
+    const int8_t *label_m, *label_f;     /* [n] phaseStat / fphaseStat */
+    const double *Po_m, *Po_f;           /* [n] oil pressure P[..][1], Pa */
+    const double *x_m, *y_m, *x_f, *y_f; /* [nc*n] comp[..][i][0..1] */
+    const double *S_m, *S_f;             /* [3*n] sat[..][0..2] */
+    const double *rho_prev_m;            /* [2*n] matrix oil/gas molar density of the
+                                            previous property sweep (CalcIFT reads them
+                                            before Calc_Visco refreshes them), or NULL:
+                                            then a first sweep with zero densities
+                                            supplies them */
+    const double *pre_m, *pre_f;         /* [(nc+1)*n] preProp/fpreProp (Pre_Mul) or NULL =
+                                            this state's accumulation (the residual at X0) */
+    double pcoeff;                       /* PCoeff (CalcPCoeff, Init_Calcs.cpp:1118-1130) */
+} osr_state_dev;
+
+typedef struct osr_asm osr_asm;
+
+/* Assembly context for one grid: derives the face kA/h (IFK_Calc,
+ * Init_Calcs.cpp:21-42), block heights (CalcBlockHeight, :993-1011), the
+ * low-pressure viscosities (Tr_Mu, :48-64) and the binary diffusion table
+ * (Mass_Tran.cpp:38-45), and allocates the property workspace. */
+int osr_asm_create(osr_ctx *ctx, const osr_fluid_ext *fe, const osr_grid *grid, osr_asm **out);
+int osr_asm_destroy(osr_asm *a);
+
+/* Block offsets of the layout for the state's labels (PJac).  offsets
+ * (device, may be NULL) receives 2n+1 entries; *nrows (host) = MatrixSize.
+ * Must be called after any label change, before residual/Jacobian. */
+int osr_asm_layout(osr_asm *a, const osr_state_dev *s, int64_t *offsets, int64_t *nrows, void *stream);
+
+/* Residual F[nrows] (device): the rows IteratorFunction writes for every
+ * block (Petsc_Functions.cpp:10-6896), after the property sweep
+ * UpdateDependencies runs (PetscAUX.cpp:500-547).  Replaces the
+ * FormFunction row loop (PetscAUX.cpp:59-104). */
+int osr_residual_device(osr_asm *a, const osr_state_dev *s, double *F, void *stream);
+
+/* Jacobian row pointer (device, nrows+1) and, on the host, the nnz.
+ * Row pattern: balance and total rows couple to every unknown of the own
+ * block, the six face neighbours and the other continuum's cell; the water
+ * row to the own block and the P, Sw columns of the others; fugacity and
+ * constraint rows to the own block; zero-porosity rows hold the diagonal. */
+int osr_jacobian_rowptr_device(osr_asm *a, const osr_state_dev *s, int64_t *rowptr, int64_t *nnz, void *stream);
+
+/* Jacobian values and column indices (device CSR, columns ascending within a
+ * row; duplicates never occur).  The values are the derivatives of the
+ * residual above in the unknowns listed, with preProp and the previous-sweep
+ * densities held fixed; zero-porosity rows are identity rows
+ * (Petsc_Functions.cpp:42051-42056).  Replaces the FormJacobian row loop
+ * (PetscAUX.cpp:106-173, IteratorJacobian Petsc_Functions.cpp:6898-42073). */
+int osr_jacobian_device(osr_asm *a, const osr_state_dev *s, const int64_t *rowptr, int32_t *cols,
+                        double *vals, void *stream);
+
+/* Cell properties after the sweep, [16+2nc][2n] (device): porpor, Pw, Po, Pg,
+ * IFT, rho_o, rho_g, mu_o, mu_g, gamma_o, gamma_g, krw, kro, krg, Pcw, Pcg,
+ * D_o[nc], D_g[nc]. */
+int osr_props_device(osr_asm *a, const osr_state_dev *s, double *out, void *stream);
+
 /* Library version string and the component counts compiled in. */
 const char *osr_version(void);
 int osr_supported_nc(int32_t nc);

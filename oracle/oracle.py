@@ -277,3 +277,31 @@ def unpack(fluid, grid, state, X):
         arr = np.ctypeslib.as_array(ptrs[k], shape=(sizes[k],)).copy()
         setattr(s2, k, arr.reshape(np.shape(getattr(state, k))))
     return s2
+
+
+def phase_ctrl(fluid, T, label, Po, Pg, x, y, S, nthreads=0):
+    """PhaseCtrl over one continuum; returns (label, x, y, S, psat) copies."""
+    f = OracleFluid(fluid)
+    label = np.array(label, dtype=np.int8, copy=True)
+    x = np.array(x, dtype=np.float64, copy=True)
+    y = np.array(y, dtype=np.float64, copy=True)
+    S = np.array(S, dtype=np.float64, copy=True)
+    n = label.shape[0]
+    Po = _c(Po)
+    Pg = _c(Pg)
+    Tarr = None if np.isscalar(T) else _c(T)
+    T0 = float(T) if np.isscalar(T) else 0.0
+    psat = np.zeros(n)
+    lib().or_phase_ctrl(f.ref, C.c_int64(n), _p(Tarr), C.c_double(T0),
+                        label.ctypes.data_as(C.c_void_p), _p(Po), _p(Pg), _p(x), _p(y), _p(S),
+                        _p(psat), C.c_int(nthreads))
+    return label, x, y, S, psat
+
+
+def residual_scale(fluid, grid, state):
+    """Per-row sum of |terms| of the residual (yardstick for parity)."""
+    a = _Asm(fluid, grid, state)
+    _, off = layout(fluid, grid, state)
+    F = np.zeros(int(off[-1]))
+    lib().or_residual_scale(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state), _p(F))
+    return F

@@ -444,6 +444,12 @@ struct Neigh {
     double h;
 };
 
+// Row-scale mode: every term enters |term| so a row's value is the sum of
+// the magnitudes it is built from (the yardstick of the parity tests).
+thread_local bool g_absmode = false;
+template <class T> inline void acc_add(T& a, const T& t) { a += g_absmode ? T(std::fabs(val(t))) : t; }
+inline double absif(double v) { return g_absmode ? std::fabs(v) : v; }
+
 // Residual rows of one block.  V/Pp are indexed [continuum][cell].
 template <class T>
 void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, const std::vector<Props<T>>* Pp,
@@ -485,7 +491,8 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
             const T FFg = log((Zg + M.d2 * Bg) / (Zg + M.d1 * Bg));
             const T phiL = exp(t1l * (Zl - 1) - log(Zl - Bl) + DDl * EEl * FFl);
             const T phiG = exp(t1g * (Zg - 1) - log(Zg - Bg) + DDg * EEg * FFg);
-            out[i] = v.x[i] * p.Po * phiL - v.y[i] * p.Pg * phiG;
+            out[i] = g_absmode ? T(std::fabs(val(v.x[i] * p.Po * phiL)) + std::fabs(val(v.y[i] * p.Pg * phiG)))
+                               : v.x[i] * p.Po * phiL - v.y[i] * p.Pg * phiG;
         }
     }
     const int r0 = two ? nc : 0;   // first component-balance row
@@ -566,12 +573,14 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
                 cg = (f.up_g == 0 ? ua_o : ub_o).y[i];
             }
             // flux into self: (P_nb - P_self) for both orientations
-            RS += co * f.to * (po.Po - p.Po) + cg * f.tg * (po.Pg - p.Pg);
-            RS += co * f.tgo * (nh[q] - hself) + cg * f.tgg * (nh[q] - hself);
+            acc_add(RS, co * f.to * (po.Po - p.Po));
+            acc_add(RS, cg * f.tg * (po.Pg - p.Pg));
+            acc_add(RS, co * f.tgo * (nh[q] - hself));
+            acc_add(RS, cg * f.tgg * (nh[q] - hself));
         }
         // convective matrix-fracture transfer
-        if (!total) RS += sgn * (vm.x[i] * convo + vm.y[i] * convg);
-        else RS += sgn * (convo + convg);
+        if (!total) { acc_add(RS, sgn * vm.x[i] * convo); acc_add(RS, sgn * vm.y[i] * convg); }
+        else { acc_add(RS, sgn * convo); acc_add(RS, sgn * convg); }
         if (!total) {
             // diffusive matrix-fracture transfer (:5259-5313 matrix rows,
             // :740-806 fracture rows); interface compositions are zero
@@ -582,29 +591,29 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
             if (k == 0) {
                 const T dxo = vf.x[i] - vm.x[i], dyo = vf.y[i] - vm.y[i];
                 const T exo = T(0) - vm.x[i], eyo = T(0) - vm.y[i];
-                if (lm == 1 && lf == 1) RS += mo * dxo;
-                if (lm == -1 && lf == -1) RS += mg * dyo;
-                if (lm == 0 && lf == 0) { RS += mo * dxo; RS += mg * dyo; }
-                if (lm == 1 && lf == -1) RS += mo * exo;
-                if (lm == -1 && lf == 1) RS += mg * eyo;
-                if (lm == 0 && lf == 1) { RS += mg * eyo; RS += mo * dxo; }
-                if (lm == 0 && lf == -1) { RS += mg * dyo; RS += mo * exo; }
-                if (lm == 1 && lf == 0) { RS += mo * exo; RS += mo * dxo; }
-                if (lm == -1 && lf == 0) { RS += mg * dyo; RS += mg * eyo; }
+                if (lm == 1 && lf == 1) acc_add(RS, mo * dxo);
+                if (lm == -1 && lf == -1) acc_add(RS, mg * dyo);
+                if (lm == 0 && lf == 0) { acc_add(RS, mo * dxo); acc_add(RS, mg * dyo); }
+                if (lm == 1 && lf == -1) acc_add(RS, mo * exo);
+                if (lm == -1 && lf == 1) acc_add(RS, mg * eyo);
+                if (lm == 0 && lf == 1) { acc_add(RS, mg * eyo); acc_add(RS, mo * dxo); }
+                if (lm == 0 && lf == -1) { acc_add(RS, mg * dyo); acc_add(RS, mo * exo); }
+                if (lm == 1 && lf == 0) { acc_add(RS, mo * exo); acc_add(RS, mo * dxo); }
+                if (lm == -1 && lf == 0) { acc_add(RS, mg * dyo); acc_add(RS, mg * eyo); }
             } else {
                 const T fo = sf3 * Vb * pf.porpor / M.tor[c] * vf.S[1] * pf.ro[0] * pf.D[0][i];
                 const T fg = sf3 * Vb * pf.porpor / M.tor[c] * vf.S[2] * pf.ro[1] * pf.D[1][i];
                 const T dxo = vf.x[i] - vm.x[i], dyo = vf.y[i] - vm.y[i];
                 const T fxo = vf.x[i] - T(0), fyo = vf.y[i] - T(0);
-                if (lm == 1 && lf == 1) RS -= mo * dxo;
-                if (lm == -1 && lf == -1) RS -= mg * dyo;
-                if (lm == 0 && lf == 0) { RS -= mo * dxo; RS -= mg * dyo; }
-                if (lm == 1 && lf == -1) RS -= fg * fyo;
-                if (lm == -1 && lf == 1) RS -= fo * fxo;
-                if (lm == 0 && lf == 1) { RS -= fo * fxo; RS -= mo * dxo; }
-                if (lm == 0 && lf == -1) { RS -= mg * dyo; RS -= fg * fyo; }
-                if (lm == 1 && lf == 0) { RS -= fg * fyo; RS -= mo * dxo; }
-                if (lm == -1 && lf == 0) { RS -= mg * dyo; RS -= fo * fxo; }
+                if (lm == 1 && lf == 1) acc_add(RS, -(mo * dxo));
+                if (lm == -1 && lf == -1) acc_add(RS, -(mg * dyo));
+                if (lm == 0 && lf == 0) { acc_add(RS, -(mo * dxo)); acc_add(RS, -(mg * dyo)); }
+                if (lm == 1 && lf == -1) acc_add(RS, -(fg * fyo));
+                if (lm == -1 && lf == 1) acc_add(RS, -(fo * fxo));
+                if (lm == 0 && lf == 1) { acc_add(RS, -(fo * fxo)); acc_add(RS, -(mo * dxo)); }
+                if (lm == 0 && lf == -1) { acc_add(RS, -(mg * dyo)); acc_add(RS, -(fg * fyo)); }
+                if (lm == 1 && lf == 0) { acc_add(RS, -(fg * fyo)); acc_add(RS, -(mo * dxo)); }
+                if (lm == -1 && lf == 0) { acc_add(RS, -(mg * dyo)); acc_add(RS, -(fo * fxo)); }
             }
             // inter-cell Fickian flux (:5346-5656): thickness-weighted
             // density x saturation over tortuosity/porosity (tortuosity 1
@@ -632,11 +641,11 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
                     const T BB = tn * dn * Cf / pp2 + ts * dl * dcs / p.porpor;
                     T Dt = AA / BB * NN * Dup;
                     if (Cf == 0 && val(dcs) == 0) Dt = T(0);
-                    RS += Dt * (xn - xs);
+                    acc_add(RS, Dt * (xn - xs));
                 }
             }
         }
-        out[r0 + i] = LS - RS;
+        out[r0 + i] = g_absmode ? T(std::fabs(val(LS)) + val(RS)) : LS - RS;
     }
     // water balance
     {
@@ -647,25 +656,25 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
             if (nb[q] < 0) continue;
             const int o = nb[q];
             const Props<T>& po = Pp[k][o];
-            RS += F6[q].tw * (po.Pw - p.Pw);
-            RS += G_ACC * M.wat_ro * F6[q].tw * (nh[q] - hself);
+            acc_add(RS, F6[q].tw * (po.Pw - p.Pw));
+            acc_add(RS, G_ACC * M.wat_ro * F6[q].tw * (nh[q] - hself));
         }
-        RS += sgn * (SI_C * M.wat_ro * WAT_M_RO * sf2 * krw_mf / M.wat_mu * Vb *
-                     ((pf.Pw - pm.Pw) + (pm.pcw - pf.pcw)));
-        out[r0 + nc] = LS - RS;
+        acc_add(RS, sgn * (SI_C * M.wat_ro * WAT_M_RO * sf2 * krw_mf / M.wat_mu * Vb *
+                     ((pf.Pw - pm.Pw) + (pm.pcw - pf.pcw))));
+        out[r0 + nc] = g_absmode ? T(std::fabs(val(LS)) + val(RS)) : LS - RS;
     }
     // local constraints
     if (two) {
         T sx = 0, sy = 0;
         for (int i = 0; i < nc; i++) { sx += v.x[i]; sy += v.y[i]; }
-        out[2 * nc + 1] = sx - 1;
-        out[2 * nc + 2] = sy - 1;
-        out[2 * nc + 3] = v.S[0] + v.S[1] + v.S[2] - 1;
+        out[2 * nc + 1] = g_absmode ? T(val(sx) + 1) : sx - 1;
+        out[2 * nc + 2] = g_absmode ? T(val(sy) + 1) : sy - 1;
+        out[2 * nc + 3] = g_absmode ? T(val(v.S[0] + v.S[1] + v.S[2]) + 1) : v.S[0] + v.S[1] + v.S[2] - 1;
     } else {
         T s = 0;
         for (int i = 0; i < nc; i++) s += (lab == 1 ? v.x[i] : v.y[i]);
-        out[nc + 1] = s - 1;
-        out[nc + 2] = v.S[0] + v.S[1] + v.S[2] - 1;
+        out[nc + 1] = g_absmode ? T(val(s) + 1) : s - 1;
+        out[nc + 2] = g_absmode ? T(val(v.S[0] + v.S[1] + v.S[2]) + 1) : v.S[0] + v.S[1] + v.S[2] - 1;
     }
 }
 
@@ -805,6 +814,8 @@ int or_layout(const or_grid* g, const or_state* s, int nc, int32_t* bsize, int64
     return 0;
 }
 
+int or_residual_scale(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const or_state* s, double* F);
+
 int or_residual(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const or_state* s, double* F) {
     Model M;
     build_model(M, f, fe, g);
@@ -822,6 +833,14 @@ int or_residual(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, con
             off += bs;
         }
     return 0;
+}
+
+// Per-row sum of the magnitudes of the terms the residual row adds up.
+int or_residual_scale(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const or_state* s, double* F) {
+    g_absmode = true;
+    const int rc = or_residual(f, fe, g, s, F);
+    g_absmode = false;
+    return rc;
 }
 
 // Cell properties (for property-level parity): per continuum and cell
@@ -920,6 +939,17 @@ int64_t or_jacobian(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g,
                     for (int j = 0; j < dbs && j < rows[r].n; j++) Jd[q][(size_t)r * dbs + j] = rows[r].d[j];
                 VD[dk][dc] = saved;
                 PD[dk][dc] = savedp;
+            }
+            if (M.phi[k][c] < 1e-10) {
+                // zero-porosity block: identity rows (Petsc_Functions.cpp:42051-42056)
+                for (int r = 0; r < bs; r++) {
+                    const int64_t row = boff[k * n + c] + r;
+                    rowptr[row] = nnz;
+                    if (nnz < cap) { cols[nnz] = (int32_t)row; vals[nnz] = 1.0; }
+                    else overflow = true;
+                    nnz++;
+                }
+                continue;
             }
             for (int r = 0; r < bs; r++) {
                 const int64_t row = boff[k * n + c] + r;

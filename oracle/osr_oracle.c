@@ -582,3 +582,74 @@ void or_psat_batch(const or_fluid *f, int64_t n, const double *T, double T0,
     }
     (void)nthreads;
 }
+
+/* ------------------------------------------------------------- PhaseCtrl */
+/* PhaseCtrl / fPhaseCtrl (Do_Cycle.cpp:1558-1643 / 1644-1722) for n cells of
+ * one continuum: gas cells whose gas pressure is at or below 0.95 x the dew
+ * pressure, and liquid cells whose oil pressure is at or below 0.95 x the
+ * bubble pressure, turn two-phase with an incipient phase of saturation
+ * EPS_SAT and the saturation-pressure routine's incipient composition; any
+ * other cell whose oil (gas) saturation is <= 0 becomes gas (liquid); the
+ * saturations are renormalised.  Arrays are SoA; label, x, y, S are updated
+ * in place; psat (may be NULL) receives the saturation pressure evaluated
+ * (-1 where none was). */
+void or_phase_ctrl(const or_fluid *f, int64_t n, const double *T, double T0,
+                   int8_t *label, const double *Po, const double *Pg,
+                   double *x, double *y, double *S, double *psat, int nthreads)
+{
+    const int nc = f->nc;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+#endif
+    for (int64_t c = 0; c < n; c++) {
+        double xc[NCMAX], yc[NCMAX], xm[NCMAX];
+        for (int i = 0; i < nc; i++) {
+            xc[i] = x[(int64_t)i * n + c];
+            yc[i] = y[(int64_t)i * n + c];
+        }
+        const double Tc = T ? T[c] : T0;
+        double *Sw = S + c, *So = S + n + c, *Sg = S + 2 * n + c;
+        double ps = -1;
+        int lab = label[c];
+        int done = 0;
+        if (lab == -1) {
+            ps = or_dew(f, Tc, Pg[c], yc, xm);
+            if (Pg[c] <= OR_PHASE_ENV * ps) {
+                *So = OR_EPS_SAT;
+                *Sg -= OR_EPS_SAT;
+                lab = 0;
+                for (int i = 0; i < nc; i++) x[(int64_t)i * n + c] = xm[i];
+                done = 1;
+            }
+        } else if (lab == 1) {
+            ps = or_bubble(f, Tc, Po[c], xc, yc, xm);
+            if (Po[c] <= OR_PHASE_ENV * ps) {
+                *Sg = OR_EPS_SAT;
+                *So -= OR_EPS_SAT;
+                lab = 0;
+                for (int i = 0; i < nc; i++) y[(int64_t)i * n + c] = xm[i];
+                done = 1;
+            }
+        }
+        if (!done) {
+            if (*So <= OR_EPS_SAT * OR_EPS_SAT_FRAC) {
+                *So = 0;
+                *Sg = 1 - *Sw;
+                lab = -1;
+            }
+            if (*Sg <= OR_EPS_SAT * OR_EPS_SAT_FRAC) {
+                *Sg = 0;
+                *So = 1 - *Sw;
+                lab = 1;
+            }
+        }
+        const double inv = 1.0 / (*Sw + *So + *Sg);
+        *Sw *= inv;
+        *So *= inv;
+        *Sg *= inv;
+        label[c] = (int8_t)lab;
+        if (psat) psat[c] = ps;
+    }
+    (void)nthreads;
+}

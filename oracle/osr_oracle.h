@@ -93,6 +93,12 @@ void or_psat_batch(const or_fluid *f, int64_t n, const double *T, double T0,
                    const double *P, const int8_t *kind, const double *x,
                    const double *y, double *psat, double *xm, int nthreads);
 
+/* PhaseCtrl / fPhaseCtrl (Do_Cycle.cpp:1558-1722) over n cells of one
+ * continuum; label, x, y, S ([3*n]: Sw, So, Sg) updated in place. */
+void or_phase_ctrl(const or_fluid *f, int64_t n, const double *T, double T0,
+                   int8_t *label, const double *Po, const double *Pg,
+                   double *x, double *y, double *S, double *psat, int nthreads);
+
 /* Dense LU with partial pivoting + solve, the arithmetic LAPACK dgesv performs
  * (called through MKLS, MKL_Solver.Cpp:66-90).  A is n*n row-major and is
  * overwritten.  Returns 0, or k>0 if U(k-1,k-1) is exactly zero. */
@@ -144,6 +150,8 @@ typedef struct {
 
 int or_layout(const or_grid *g, const or_state *s, int nc, int32_t *bsize, int64_t *offset);
 int or_residual(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *F);
+/* Same rows with every term taken by magnitude: the scale of each row. */
+int or_residual_scale(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *F);
 int or_props(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *out);
 int64_t or_jacobian(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s,
                     int64_t *rowptr, int32_t *cols, double *vals, int64_t cap);

@@ -1,0 +1,1257 @@
+// assembly.hip -- residual and Jacobian assembly of the dual-porosity
+// compositional model on gfx950, in the reference's variable-block AIJ
+// layout (matrix blocks, then fracture blocks; block sizes nc+3 for single-
+// phase and 2nc+4 for two-phase cells).
+//
+// Reference arithmetic restated (compsim/...):
+//   property sweep  PetscAUX.cpp:500-547 order: CalcIFT Do_Cycle.cpp:2438-2456,
+//                   CalcSatFs :99-259 / fCalcSatFs :261-437, Calc_Phase_P
+//                   :1418-1446, Calc_Visco :438-616 / fCalc_Visco :619-799,
+//                   CalcDiffusion Mass_Tran.cpp:21-62, CalcPhaseGrav
+//                   Do_Cycle.cpp:2022-2057, Pre_Mul Do_Time_Step.cpp:54-84
+//   face terms      Calc_Transes Do_Cycle.cpp:800-1108 / fCalc_Transes :1109-1417
+//   residual rows   IteratorFunction Petsc_Functions.cpp:10-6896
+//   layout          PJac Init_Calcs.cpp:1013-1116
+//
+// GPU structure: one lane per block (continuum, cell).  k_props writes the
+// cell properties structure-of-arrays ([field][2n]) so the stencil reads of
+// the residual kernel are coalesced along x.  The residual of a block is the
+// sum of separable pieces -- accumulation, six face fluxes, the matrix-
+// fracture transfer, the fugacity and constraint rows -- and the Jacobian
+// kernel differentiates only the pieces that touch the dependency block it
+// owns (one lane per row block x dependency block x chunk of W unknowns),
+// with forward-mode duals in registers; it writes its entries straight into
+// the CSR arrays, so no atomics and the result is bitwise reproducible.
+#include <hipcub/hipcub.hpp>
+
+#include "dual_dev.hpp"
+#include "osr_internal.hpp"
+
+namespace osr {
+
+constexpr double SI_C = 9.869232667160128e-13;   // Functions.h:207
+constexpr double WAT_M_RO = 55.5084;             // Functions.h:206
+constexpr double G_ACC = 9.80665;                // Functions.h:198
+constexpr double G_ACCG = 9.80665e-3;            // Functions.h:199
+constexpr double IFTBASECASE = 50;               // Functions.h:279
+constexpr double MISCIBLEIFT = 1;                // Functions.h:281
+constexpr double IFTPOWER = 7;                   // Functions.h:280
+constexpr double RELPERM0 = 0;                   // Functions.h:266
+constexpr int MAXTAB = 32;
+
+// Constants of one assembly context: EOS at the run temperature, the
+// component data the property correlations need and the binary diffusion
+// table (all T-only), read with wave-uniform loads.
+struct AsmConsts {
+    int nc, eos;
+    double T, d1, d2;
+    double a[NCMAX], b[NCMAX], sa[NCMAX];
+    double bic[NCMAX * NCMAX];
+    double mw[NCMAX], sqmw[NCMAX], vcm3[NCMAX], mu0[NCMAX], tc[NCMAX], pc[NCMAX], par[NCMAX];
+    double v53[NCMAX], v23[NCMAX];
+    double pdij[NCMAX * NCMAX];   // pDij (Mass_Tran.cpp:41-45), 0 on the diagonal
+    double pdlast[NCMAX];         // pDij of the last k != j (used when x_j == 1)
+    double swt[4 * MAXTAB], sgt[4 * MAXTAB];
+    int nswt, nsgt;
+    double refP, cpor, dcpor, dt, wat_ro, wat_mu, sigma, lf, cdiff;
+    int const_diff;
+};
+
+struct AsmArgs {
+    int nx, ny, nz, n;
+    const AsmConsts* K;
+    const double *dx, *dy, *dz, *h;
+    const double *phi;       // [2n]
+    const double *perm_m;    // [3n]
+    const double *iftran;    // [2][3][n]
+    const double *tor;       // [n]
+    // state
+    const int8_t* lab[2];
+    const double* Po[2];
+    const double* x[2];
+    const double* y[2];
+    const double* S[2];
+    const double* rho_prev;  // [2n] or null
+    const double* pre[2];    // [(nc+1) n] or null
+    double pcoeff;
+    // derived
+    double* props;           // [NF][2n]
+    const int64_t* boff;     // [2n+1]
+};
+
+__device__ __forceinline__ int nfields(int nc) { return 16 + 2 * nc; }
+
+// ---------------------------------------------------------------- EOS (T)
+template <class T>
+__device__ __forceinline__ T zroot(T a1, T a2, T a3, bool gas) {
+    const T Q = (3.0 * a2 - a1 * a1) / 9.0;
+    const T J = (9.0 * a1 * a2 - 27.0 * a3 - 2.0 * a1 * a1 * a1) / 54.0;
+    const T D = Q * Q * Q + J * J;
+    T Z;
+    if (D < 0) {
+        const T r = 2.0 * fsqrt(-Q);
+        const T s = a1 / 3.0;
+        const T t = facos(J / fsqrt(-Q * Q * Q)) / 3.0;
+        const T x1 = r * fcos(t) - s;
+        const T x2 = r * fcos(t + 2.0 * 3.1415926535897932384626433 / 3) - s;
+        const T x3 = r * fcos(t + 4.0 * 3.1415926535897932384626433 / 3) - s;
+        if (gas) {
+            if (x1 >= x2 && x1 >= x3) Z = x1;
+            else if (x2 >= x1 && x2 >= x3) Z = x2;
+            else Z = x3;
+        } else {
+            if (x1 <= x2 && x1 <= x3 && x1 > 0) Z = x1;
+            else if (x2 <= x1 && x2 <= x3 && x2 > 0) Z = x2;
+            else Z = x3;
+        }
+    } else if (D == 0) {
+        const T t = fcbrt(J);
+        const T x1 = 2.0 * t - a1 / 3.0;
+        const T x2 = -t - a1 / 3.0;
+        if (gas) Z = (x2 >= x1) ? x2 : x1;
+        else Z = (x2 <= x1 && x2 > 0) ? x2 : x1;
+    } else {
+        const T sd = fsqrt(D);
+        Z = fcbrt(J + sd) + fcbrt(J - sd) - a1 / 3.0;
+    }
+    return Z;
+}
+
+template <int NC, class T>
+__device__ __forceinline__ void mixT(const AsmConsts& K, const T* x, T& Ca, T& Cb, T* t) {
+    T u[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) u[j] = x[j] * K.sa[j];
+    Ca = 0.0;
+    Cb = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        T s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NC; j++) s += u[j] * K.bic[i * NC + j];
+        Ca += u[i] * s;
+        Cb += x[i] * K.b[i];
+        if (t) t[i] = s * K.sa[i];
+    }
+}
+
+template <int NC, class T>
+__device__ __forceinline__ T zphase(const AsmConsts& K, const T* x, const T& P, bool gas) {
+    T Ca, Cb;
+    mixT<NC, T>(K, x, Ca, Cb, (T*)nullptr);
+    const double rt = RGAS * K.T;
+    const T A = Ca * P / (rt * rt);
+    const T B = Cb * P / rt;
+    const bool srk = K.eos == OSR_EOS_SRK;
+    const T U = srk ? B : 2.0 * B;
+    const T W = srk ? T(0.0) : B;
+    return zroot<T>(-(1.0 + B - U), A - B * U - U - W * W, -(A * B - B * W * W - W * W), gas);
+}
+
+// ------------------------------------------------------- cell properties
+template <int NC, class T>
+struct PropsT {
+    T porpor, Pw, Po, Pg, bift;
+    T ro[2], mu[2], gam[2];
+    T kr[3], pcw, pcg;
+    T D[2][NC];
+};
+
+template <class T>
+__device__ __forceinline__ void table(const double* tb, int nr, const T& s, T& kr, T& kro, T& pc) {
+    int r = 0;
+    while (r < nr && !(s < tb[4 * r])) r++;
+    if (r == 0) {
+        kr = T(RELPERM0);
+        pc = T(tb[3]);
+        kro = T(tb[2]);
+    } else if (r == nr) {
+        kr = T(tb[4 * (nr - 1) + 1]);
+        pc = T(tb[4 * (nr - 1) + 3]);
+        kro = T(tb[4 * (nr - 1) + 2]);
+    } else {
+        const double s0 = tb[4 * (r - 1)], s1 = tb[4 * r];
+        const double inv = 1.0 / (s1 - s0);
+        kr = (tb[4 * r + 1] - tb[4 * (r - 1) + 1]) * inv * (s - s1) + tb[4 * r + 1];
+        kro = (tb[4 * r + 2] - tb[4 * (r - 1) + 2]) * inv * (s - s1) + tb[4 * r + 2];
+        pc = (tb[4 * r + 3] - tb[4 * (r - 1) + 3]) * inv * (s - s1) + tb[4 * r + 3];
+    }
+}
+
+template <int NC, class T>
+__device__ __forceinline__ void viscoT(const AsmConsts& K, const T* x, const T& P, bool gas, T& ro, T& mu) {
+    const T Z = zphase<NC, T>(K, x, P, gas);
+    ro = P / (Z * (RGAS * K.T));
+    T s1 = 0.0, s2 = 0.0, rv = 0.0, tcm = 0.0, pcm = 0.0, mwm = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        const T tm = x[i] * K.sqmw[i];
+        s1 += tm * K.mu0[i];
+        s2 += tm;
+        rv += x[i] * K.vcm3[i];
+        tcm += x[i] * K.tc[i];
+        pcm += x[i] * (K.pc[i] / 101325);
+        mwm += x[i] * K.mw[i];
+    }
+    const T Er = ro * rv;
+    const T etha = fpow(tcm, 1.0 / 6) / (fsqrt(mwm) * fpow(pcm, 2.0 / 3));
+    const T r1 = 0.10230 + 0.023364 * Er + 0.058533 * Er * Er - 0.040758 * Er * Er * Er +
+                 0.0093324 * Er * Er * Er * Er;
+    const T r2 = r1 * r1;
+    mu = s1 / s2 + (r2 * r2 - 1e-4) / etha;
+}
+
+template <int NC, class T>
+__device__ __forceinline__ void diffT(const AsmConsts& K, const T* x, const T& ro, T* D) {
+    T s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        s1 += x[j] * K.v53[j];
+        s2 += x[j] * K.v23[j];
+    }
+    const T rr = ro * s1 / s2;
+    const T roRD = (0.99589 + 0.096016 * rr - 0.22035 * rr * rr + 0.032874 * rr * rr * rr) / ro;
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        T s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (k == j) continue;
+            s += x[k] / (K.pdij[j * NC + k] * roRD);
+        }
+        const T dt = (1.0 - x[j]) / (s * 10000.0);
+        T dv;
+        if (x[j] == 1.0) dv = roRD * K.pdlast[j];
+        else if (dt < 0 || s == 0.0) dv = T(0.0);
+        else dv = dt;
+        if (K.const_diff) dv = T(K.cdiff);
+        D[j] = dv;
+    }
+}
+
+template <int NC, class T>
+__device__ __forceinline__ void propsT(const AsmArgs& A, int k, int c, int lab, const T& Po, const T* x,
+                                       const T* y, const T* S, double rpo, double rpg, PropsT<NC, T>& p) {
+    const AsmConsts& K = *A.K;
+    const double phi = A.phi[k * A.n + c];
+    const T dP = Po - K.refP;
+    p.Po = Po;
+    p.porpor = phi * (1.0 + K.cpor * dP + K.dcpor * dP * dP);
+    if (k == 0) {
+        if (lab == 0) {
+            T ps = 0.0;
+#pragma unroll
+            for (int i = 0; i < NC; i++) ps += K.par[i] * (x[i] * rpo - y[i] * rpg);
+            p.bift = fpow(ps * 1e-6, 3.88);
+        } else {
+            p.bift = T(IFTBASECASE);
+        }
+        T krw, krow, pcw, krg, krog, pcg;
+        table<T>(K.swt, K.nswt, S[0], krw, krow, pcw);
+        table<T>(K.sgt, K.nsgt, S[2], krg, krog, pcg);
+        const double kro0 = K.swt[2];
+        const T AA = krw + krow / kro0;
+        const T BB = krg + krog / kro0;
+        T kro = kro0 * (AA * BB - krw - krg);
+        if (p.bift < MISCIBLEIFT) {
+            const T F = fpow(p.bift / IFTBASECASE, 1.0 / IFTPOWER);
+            kro = F * kro + (1.0 - F) * S[1] * (1 - K.swt[0]);
+            krg = F * krg + (1.0 - F) * S[2] * (1 - K.swt[0]);
+        }
+        if (kro < 0) kro = T(RELPERM0);
+        if (kro > 1) kro = T(1.0);
+        p.kr[0] = krw;
+        p.kr[1] = kro;
+        p.kr[2] = krg;
+        p.pcw = pcw;
+        p.pcg = pcg;
+        p.Pw = Po - pcw;
+        if (p.bift < MISCIBLEIFT) p.Pg = Po + pcg * p.bift / IFTBASECASE;
+        else p.Pg = Po + pcg;
+    } else {
+        p.bift = T(IFTBASECASE);
+        p.kr[0] = S[0];
+        p.kr[1] = S[1];
+        p.kr[2] = S[2];
+        p.pcw = T(0.0);
+        p.pcg = T(0.0);
+        p.Pw = Po;
+        p.Pg = Po;
+    }
+    viscoT<NC, T>(K, x, p.Po, false, p.ro[0], p.mu[0]);
+    viscoT<NC, T>(K, y, p.Pg, true, p.ro[1], p.mu[1]);
+    diffT<NC, T>(K, x, p.ro[0], p.D[0]);
+    diffT<NC, T>(K, y, p.ro[1], p.D[1]);
+    T ml = 0.0, mg = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        ml += x[i] * K.mw[i];
+        mg += y[i] * K.mw[i];
+    }
+    p.gam[0] = G_ACCG * p.ro[0] * ml;
+    p.gam[1] = G_ACCG * p.ro[1] * mg;
+}
+
+// state of one block as T
+template <int NC, class T>
+struct VarsT {
+    int lab;
+    T Po;
+    T x[NC], y[NC], S[3];
+};
+
+template <int NC>
+__device__ __forceinline__ void load_vars(const AsmArgs& A, int k, int c, VarsT<NC, double>& v) {
+    const int n = A.n;
+    v.lab = A.lab[k][c];
+    v.Po = A.Po[k][c];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        v.x[i] = A.x[k][(int64_t)i * n + c];
+        v.y[i] = A.y[k][(int64_t)i * n + c];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) v.S[j] = A.S[k][(int64_t)j * n + c];
+}
+
+template <int NC>
+__device__ __forceinline__ void load_props(const AsmArgs& A, int k, int c, PropsT<NC, double>& p) {
+    const int64_t s = 2 * (int64_t)A.n;
+    const double* q = A.props + (int64_t)k * A.n + c;
+    p.porpor = q[0 * s]; p.Pw = q[1 * s]; p.Po = q[2 * s]; p.Pg = q[3 * s]; p.bift = q[4 * s];
+    p.ro[0] = q[5 * s]; p.ro[1] = q[6 * s]; p.mu[0] = q[7 * s]; p.mu[1] = q[8 * s];
+    p.gam[0] = q[9 * s]; p.gam[1] = q[10 * s];
+    p.kr[0] = q[11 * s]; p.kr[1] = q[12 * s]; p.kr[2] = q[13 * s]; p.pcw = q[14 * s]; p.pcg = q[15 * s];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        p.D[0][i] = q[(16 + i) * s];
+        p.D[1][i] = q[(16 + NC + i) * s];
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ void rho_prev_of(const AsmArgs& A, int c, double& ro, double& rg) {
+    if (A.rho_prev) {
+        ro = A.rho_prev[c];
+        rg = A.rho_prev[A.n + c];
+    } else {
+        ro = rg = 0;
+    }
+}
+
+// k_props: one lane per block; pass 0 (without previous densities) writes the
+// zero-density sweep whose densities CalcIFT reads in pass 1.
+template <int NC>
+__global__ __launch_bounds__(256) void k_props(AsmArgs A) {
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= 2 * A.n) return;
+    const int k = blk / A.n, c = blk % A.n;
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    double ro = 0, rg = 0;
+    rho_prev_of<NC>(A, c, ro, rg);
+    PropsT<NC, double> p;
+    propsT<NC, double>(A, k, c, v.lab, v.Po, v.x, v.y, v.S, ro, rg, p);
+    const int64_t s = 2 * (int64_t)A.n;
+    double* q = A.props + blk;
+    q[0 * s] = p.porpor; q[1 * s] = p.Pw; q[2 * s] = p.Po; q[3 * s] = p.Pg; q[4 * s] = p.bift;
+    q[5 * s] = p.ro[0]; q[6 * s] = p.ro[1]; q[7 * s] = p.mu[0]; q[8 * s] = p.mu[1];
+    q[9 * s] = p.gam[0]; q[10 * s] = p.gam[1];
+    q[11 * s] = p.kr[0]; q[12 * s] = p.kr[1]; q[13 * s] = p.kr[2]; q[14 * s] = p.pcw; q[15 * s] = p.pcg;
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        q[(16 + i) * s] = p.D[0][i];
+        q[(16 + NC + i) * s] = p.D[1][i];
+    }
+}
+
+// Copy the first-pass matrix densities into the [2n] previous-density buffer.
+__global__ void k_take_rho(const double* props, int n, double* rp) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int64_t s = 2 * (int64_t)n;
+    rp[c] = props[5 * s + c];
+    rp[n + c] = props[6 * s + c];
+}
+
+// ------------------------------------------------------------ geometry
+struct Geo {
+    int ix, iy, iz;
+    int nb[6];          // neighbour cells +x -x +y -y +z -z, -1 outside
+    double dsz[3];      // own sizes
+    double nd[6];       // neighbour sizes along the direction
+    double nh[6];       // neighbour block heights
+    double area[6];     // face areas
+    double h, Vb;
+};
+
+__device__ __forceinline__ Geo geom(const AsmArgs& A, int c) {
+    Geo g;
+    const int nxy = A.nx * A.ny;
+    g.iz = c / nxy;
+    const int rem = c - g.iz * nxy;
+    g.iy = rem / A.nx;
+    g.ix = rem - g.iy * A.nx;
+    g.dsz[0] = A.dx[g.ix];
+    g.dsz[1] = A.dy[g.iy];
+    g.dsz[2] = A.dz[g.iz];
+    g.h = A.h[g.iz];
+    g.Vb = g.dsz[0] * g.dsz[1] * g.dsz[2];
+    g.nb[0] = g.ix < A.nx - 1 ? c + 1 : -1;
+    g.nb[1] = g.ix > 0 ? c - 1 : -1;
+    g.nb[2] = g.iy < A.ny - 1 ? c + A.nx : -1;
+    g.nb[3] = g.iy > 0 ? c - A.nx : -1;
+    g.nb[4] = g.iz < A.nz - 1 ? c + nxy : -1;
+    g.nb[5] = g.iz > 0 ? c - nxy : -1;
+    g.nd[0] = g.nb[0] >= 0 ? A.dx[g.ix + 1] : 0;
+    g.nd[1] = g.nb[1] >= 0 ? A.dx[g.ix - 1] : 0;
+    g.nd[2] = g.nb[2] >= 0 ? A.dy[g.iy + 1] : 0;
+    g.nd[3] = g.nb[3] >= 0 ? A.dy[g.iy - 1] : 0;
+    g.nd[4] = g.nb[4] >= 0 ? A.dz[g.iz + 1] : 0;
+    g.nd[5] = g.nb[5] >= 0 ? A.dz[g.iz - 1] : 0;
+    for (int q = 0; q < 4; q++) g.nh[q] = g.h;
+    g.nh[4] = g.nb[4] >= 0 ? A.h[g.iz + 1] : 0;
+    g.nh[5] = g.nb[5] >= 0 ? A.h[g.iz - 1] : 0;
+    g.area[0] = g.area[1] = g.dsz[1] * g.dsz[2];
+    g.area[2] = g.area[3] = g.dsz[0] * g.dsz[2];
+    g.area[4] = g.area[5] = g.dsz[0] * g.dsz[1];
+    return g;
+}
+
+// ------------------------------------------------------- residual pieces
+// All pieces add into rows[] (the block's residual, F = LSide - RSide, so
+// inflow terms are subtracted).  r0: first component-balance row.
+
+// Flux across face q between self (s) and neighbour (o): convective and
+// gravity terms of every component, total and water row, and the Fickian
+// flux of each component (Calc_Transes + IteratorFunction face terms).
+template <int NC, class TS, class TO>
+__device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G, int q, int c, int o,
+                                           const VarsT<NC, TS>& vs, const PropsT<NC, TS>& ps,
+                                           const VarsT<NC, TO>& vo, const PropsT<NC, TO>& po, int r0,
+                                           decltype(TS() + TO())* rows) {
+    using T = decltype(TS() + TO());
+    const AsmConsts& K = *A.K;
+    const int d = q >> 1;
+    const bool self_is_a = (q & 1) == 0;
+    const int ca = self_is_a ? c : o;
+    const double Kf = A.iftran[((int64_t)k * 3 + d) * A.n + ca];
+    const double da = self_is_a ? G.dsz[d] : G.nd[q];
+    const double db = self_is_a ? G.nd[q] : G.dsz[d];
+    const double ha = self_is_a ? G.h : G.nh[q];
+    const double hb = self_is_a ? G.nh[q] : G.h;
+    const double dh = G.nh[q] - G.h;
+    // water (upstream by water potential)
+    {
+        const T pot = (self_is_a ? (ps.Pw - po.Pw) : (po.Pw - ps.Pw)) + G_ACC * K.wat_ro * (ha - hb);
+        const bool up_a = pot > 0;
+        const bool up_self = (up_a == self_is_a);
+        const T tw = (Kf * (K.wat_ro * WAT_M_RO / K.wat_mu)) * (up_self ? T(ps.kr[0]) : T(po.kr[0]));
+        const T fw = tw * (po.Pw - ps.Pw) + G_ACC * K.wat_ro * tw * dh;
+        rows[r0 + NC] -= fw;
+    }
+    // oil and gas
+#pragma unroll
+    for (int ph = 0; ph < 2; ph++) {
+        const int absent = ph == 0 ? -1 : 1;
+        const double cs = vs.lab == absent ? 0.0 : 1.0;
+        const double co = vo.lab == absent ? 0.0 : 1.0;
+        const double c1 = self_is_a ? cs : co, c2 = self_is_a ? co : cs;
+        const T gs = T(ps.gam[ph]);
+        const T go = T(po.gam[ph]);
+        const T gaa = self_is_a ? gs : go, gbb = self_is_a ? go : gs;
+        T mg;
+        if (c1 == 0 && c2 == 0) mg = T(0.0);
+        else mg = (gaa * da * c1 + gbb * db * c2) / (c1 * da + db * c2);
+        const TS Psel = ph == 0 ? ps.Po : ps.Pg;
+        const TO Poth = ph == 0 ? po.Po : po.Pg;
+        const T pot = (self_is_a ? (Psel - Poth) : (Poth - Psel)) + mg * (ha - hb);
+        const bool up_a = pot > 0;
+        const bool up_self = (up_a == self_is_a);
+        const T ros = T(ps.ro[ph]), roo = T(po.ro[ph]);
+        const T mus = T(ps.mu[ph]), muo = T(po.mu[ph]);
+        const T kru = up_self ? T(ps.kr[1 + ph]): T(po.kr[1 + ph]);
+        const T roa = self_is_a ? ros : roo, rob = self_is_a ? roo : ros;
+        const T mua = self_is_a ? mus : muo, mub = self_is_a ? muo : mus;
+        T t = Kf * (roa * da * c1 + rob * db * c2) / (mua * da * c1 + mub * db * c2) * kru;
+        T tg = mg * t;
+        if (c1 == 0 && c2 == 0) {
+            t = T(0.0);
+            tg = T(0.0);
+        }
+        const T dPp = Poth - Psel;
+        const T base = t * dPp + tg * dh;
+        // total hydrocarbon row
+        rows[r0 + NC - 1] -= base;
+#pragma unroll
+        for (int i = 0; i < NC - 1; i++) {
+            const T cu = up_self ? (ph == 0 ? T(vs.x[i]): T(vs.y[i]))
+                                 : (ph == 0 ? T(vo.x[i]): T(vo.y[i]));
+            rows[r0 + i] -= cu * base;
+        }
+    }
+    // Fickian inter-cell flux of the components (tortuosity 1 in the fracture)
+    const double dl = G.dsz[d], dn = G.nd[q];
+    const double NN = 2.0 * G.area[q] / (dn + dl);
+    const double tn = k == 0 ? A.tor[o] : 1.0, ts = k == 0 ? A.tor[c] : 1.0;
+    const bool zerophi = A.phi[k * A.n + c] < 1e-10 || A.phi[k * A.n + o] < 1e-10;
+#pragma unroll
+    for (int ph = 0; ph < 2; ph++) {
+        const int absent = ph == 0 ? -1 : 1;
+        const double Cf = vo.lab == absent ? 0.0 : 1.0;
+        const double dcs = vs.lab == absent ? 0.0 : 1.0;
+        const T AA = (po.ro[ph] * (dn * Cf) + ps.ro[ph] * (dl * dcs)) *
+                     ((vo.S[1 + ph] * (dn * Cf) + vs.S[1 + ph] * (dl * dcs)) / (dn + dl));
+        const T BB = (tn * dn * Cf) / po.porpor + (ts * dl * dcs) / ps.porpor;
+        const T MN = AA / BB * NN;
+#pragma unroll
+        for (int i = 0; i < NC - 1; i++) {
+            const TS xs = ph == 0 ? vs.x[i] : vs.y[i];
+            const TO xn = ph == 0 ? vo.x[i] : vo.y[i];
+            T Dup = (xn > xs) ? T(po.D[ph][i]): T(ps.D[ph][i]);
+            if (zerophi) Dup = T(0.0);
+            T Dt = MN * Dup;
+            if (Cf == 0 && dcs == 0) Dt = T(0.0);
+            rows[r0 + i] -= Dt * (xn - xs);
+        }
+    }
+}
+
+// Matrix-fracture transfer, as seen from continuum k (sign + for matrix
+// rows, - for fracture rows), Petsc_Functions.cpp:5257-5315 / :731-806.
+template <int NC, class TM, class TF>
+__device__ __forceinline__ void transfer_piece(const AsmArgs& A, int k, const Geo& G, int c,
+                                               const VarsT<NC, TM>& vm, const PropsT<NC, TM>& pm,
+                                               const VarsT<NC, TF>& vf, const PropsT<NC, TF>& pf, int r0,
+                                               decltype(TM() + TF())* rows) {
+    using T = decltype(TM() + TF());
+    const AsmConsts& K = *A.K;
+    const int n = A.n;
+    const double Vb = G.Vb;
+    const double dx = G.dsz[0], dy = G.dsz[1], dz = G.dsz[2];
+    const double sf2 = 4 * K.sigma * (A.perm_m[c] / (dx * dx) + A.perm_m[n + c] / (dy * dy) +
+                                      A.perm_m[2 * n + c] / (dz * dz));
+    const bool off = K.const_diff && K.cdiff == 0;
+    const double sf1 = off ? 0 : 4 * K.sigma * (1 / (dx * dx) + 1 / (dy * dy) + 1 / (dz * dz));
+    const double sf3 = off ? 0 : 4 * K.sigma * ((1 / dx) + (1 / dy) + (1 / dz)) / K.lf;
+    const double sgn = k == 0 ? 1.0 : -1.0;
+    const T kro = pm.Po >= pf.Po ? T(pm.kr[1]): T(pf.kr[1]);
+    const T krg = pm.Pg >= pf.Pg ? T(pm.kr[2]): T(pf.kr[2]);
+    const T krw = pm.Pw >= pf.Pw ? T(pm.kr[0]): T(pf.kr[0]);
+    const T sp = pm.porpor + pf.porpor;
+    const T muo = (pm.mu[0] * pm.porpor + pf.mu[0] * pf.porpor) / sp;
+    const T mug = (pm.mu[1] * pm.porpor + pf.mu[1] * pf.porpor) / sp;
+    const T convo = (SI_C * sf2 * Vb) * pm.ro[0] * kro / muo * (pf.Po - pm.Po);
+    const T convg = (SI_C * sf2 * Vb) * pm.ro[1] * krg / mug * ((pf.Pg - pm.Pg) + (pf.pcg - pm.pcg));
+    rows[r0 + NC - 1] -= sgn * (convo + convg);
+    const T convw = (SI_C * K.wat_ro * WAT_M_RO * sf2 * Vb / K.wat_mu) * krw * ((pf.Pw - pm.Pw) + (pm.pcw - pf.pcw));
+    rows[r0 + NC] -= sgn * convw;
+    const int lm = vm.lab, lf = vf.lab;
+    const double tor = A.tor[c];
+    const T mo0 = (sf1 * Vb / tor) * pm.porpor * vm.S[1] * pm.ro[0];
+    const T mg0 = (sf1 * Vb / tor) * pm.porpor * vm.S[2] * pm.ro[1];
+    const T fo0 = (sf3 * Vb / tor) * pf.porpor * vf.S[1] * pf.ro[0];
+    const T fg0 = (sf3 * Vb / tor) * pf.porpor * vf.S[2] * pf.ro[1];
+#pragma unroll
+    for (int i = 0; i < NC - 1; i++) {
+        T rs = (vm.x[i] * convo + vm.y[i] * convg) * sgn;
+        const T mo = mo0 * pm.D[0][i], mg = mg0 * pm.D[1][i];
+        const T dxo = vf.x[i] - vm.x[i], dyo = vf.y[i] - vm.y[i];
+        if (k == 0) {
+            const T exo = T(-vm.x[i]), eyo = T(-vm.y[i]);
+            if (lm == 1 && lf == 1) rs += mo * dxo;
+            if (lm == -1 && lf == -1) rs += mg * dyo;
+            if (lm == 0 && lf == 0) { rs += mo * dxo; rs += mg * dyo; }
+            if (lm == 1 && lf == -1) rs += mo * exo;
+            if (lm == -1 && lf == 1) rs += mg * eyo;
+            if (lm == 0 && lf == 1) { rs += mg * eyo; rs += mo * dxo; }
+            if (lm == 0 && lf == -1) { rs += mg * dyo; rs += mo * exo; }
+            if (lm == 1 && lf == 0) { rs += mo * exo; rs += mo * dxo; }
+            if (lm == -1 && lf == 0) { rs += mg * dyo; rs += mg * eyo; }
+        } else {
+            const T fo = fo0 * pf.D[0][i], fg = fg0 * pf.D[1][i];
+            const T fxo = T(vf.x[i]), fyo = T(vf.y[i]);
+            if (lm == 1 && lf == 1) rs -= mo * dxo;
+            if (lm == -1 && lf == -1) rs -= mg * dyo;
+            if (lm == 0 && lf == 0) { rs -= mo * dxo; rs -= mg * dyo; }
+            if (lm == 1 && lf == -1) rs -= fg * fyo;
+            if (lm == -1 && lf == 1) rs -= fo * fxo;
+            if (lm == 0 && lf == 1) { rs -= fo * fxo; rs -= mo * dxo; }
+            if (lm == 0 && lf == -1) { rs -= mg * dyo; rs -= fg * fyo; }
+            if (lm == 1 && lf == 0) { rs -= fg * fyo; rs -= mo * dxo; }
+            if (lm == -1 && lf == 0) { rs -= mg * dyo; rs -= fo * fxo; }
+        }
+        rows[r0 + i] -= rs;
+    }
+}
+
+// Accumulation (LSide) of the balances, the fugacity rows of a two-phase
+// block and the constraint rows: everything that depends on the block's own
+// unknowns only.
+template <int NC, class T>
+__device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const Geo& G, const VarsT<NC, T>& v,
+                                          const PropsT<NC, T>& p, int r0, T* rows) {
+    const AsmConsts& K = *A.K;
+    const int n = A.n;
+    const double f = G.Vb / K.dt;
+    const double* pre = A.pre[k];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        T acc;
+        if (i < NC - 1) acc = p.porpor * (v.x[i] * p.ro[0] * v.S[1] + v.y[i] * p.ro[1] * v.S[2]);
+        else acc = p.porpor * (p.ro[0] * v.S[1] + p.ro[1] * v.S[2]);
+        const double pr = pre ? pre[(int64_t)i * n + c] : dval(acc);
+        rows[r0 + i] += f * (acc - pr);
+    }
+    {
+        const T acc = p.porpor * (K.wat_ro * WAT_M_RO) * v.S[0];
+        const double pr = pre ? pre[(int64_t)NC * n + c] : dval(acc);
+        rows[r0 + NC] += f * (acc - pr);
+    }
+    if (v.lab == 0) {
+        T Cal, Cbl, Cag, Cbg, tl[NC], tg[NC];
+        mixT<NC, T>(K, v.x, Cal, Cbl, tl);
+        mixT<NC, T>(K, v.y, Cag, Cbg, tg);
+        const double rt = RGAS * K.T;
+        const T Al = Cal * p.Po / (rt * rt), Bl = Cbl * p.Po / rt;
+        const T Ag = Cag * p.Pg / (rt * rt), Bg = Cbg * p.Pg / rt;
+        const T Zl = p.Po / (p.ro[0] * rt), Zg = p.Pg / (p.ro[1] * rt);
+        const T DDl = Al / (Bl * (K.d1 - K.d2)), DDg = Ag / (Bg * (K.d1 - K.d2));
+        const T FFl = flog((Zl + K.d2 * Bl) / (Zl + K.d1 * Bl));
+        const T FFg = flog((Zg + K.d2 * Bg) / (Zg + K.d1 * Bg));
+        const T lzl = flog(Zl - Bl), lzg = flog(Zg - Bg);
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            const T t1l = K.b[i] / Cbl, t1g = K.b[i] / Cbg;
+            const T EEl = 2.0 * tl[i] / Cal - t1l, EEg = 2.0 * tg[i] / Cag - t1g;
+            const T phiL = fexp(t1l * (Zl - 1.0) - lzl + DDl * EEl * FFl);
+            const T phiG = fexp(t1g * (Zg - 1.0) - lzg + DDg * EEg * FFg);
+            rows[i] += v.x[i] * p.Po * phiL - v.y[i] * p.Pg * phiG;
+        }
+        T sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            sx += v.x[i];
+            sy += v.y[i];
+        }
+        rows[2 * NC + 1] += sx - 1.0;
+        rows[2 * NC + 2] += sy - 1.0;
+        rows[2 * NC + 3] += v.S[0] + v.S[1] + v.S[2] - 1.0;
+    } else {
+        T s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NC; i++) s += (v.lab == 1 ? v.x[i] : v.y[i]);
+        rows[NC + 1] += s - 1.0;
+        rows[NC + 2] += v.S[0] + v.S[1] + v.S[2] - 1.0;
+    }
+}
+
+// ------------------------------------------------------ residual kernel
+template <int NC>
+__global__ __launch_bounds__(256) void k_residual(AsmArgs A, double* F) {
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= 2 * A.n) return;
+    const int n = A.n;
+    const int k = blk / n, c = blk % n;
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    const int bs = v.lab == 0 ? 2 * NC + 4 : NC + 3;
+    double* out = F + A.boff[blk];
+    if (A.phi[k * n + c] < 1e-10) {
+        for (int r = 0; r < bs; r++) out[r] = 0;
+        return;
+    }
+    const int r0 = v.lab == 0 ? NC : 0;
+    double rows[2 * NC + 4];
+#pragma unroll
+    for (int r = 0; r < 2 * NC + 4; r++) rows[r] = 0;
+    const Geo G = geom(A, c);
+    PropsT<NC, double> p;
+    load_props<NC>(A, k, c, p);
+    own_piece<NC, double>(A, k, c, G, v, p, r0, rows);
+    for (int q = 0; q < 6; q++) {
+        const int o = G.nb[q];
+        if (o < 0) continue;
+        VarsT<NC, double> vo;
+        PropsT<NC, double> po;
+        load_vars<NC>(A, k, o, vo);
+        load_props<NC>(A, k, o, po);
+        face_piece<NC, double, double>(A, k, G, q, c, o, v, p, vo, po, r0, rows);
+    }
+    {
+        VarsT<NC, double> vx;
+        PropsT<NC, double> px;
+        load_vars<NC>(A, 1 - k, c, vx);
+        load_props<NC>(A, 1 - k, c, px);
+        if (k == 0) transfer_piece<NC, double, double>(A, 0, G, c, v, p, vx, px, r0, rows);
+        else transfer_piece<NC, double, double>(A, 1, G, c, vx, px, v, p, r0, rows);
+    }
+    for (int r = 0; r < bs; r++) out[r] = rows[r];
+}
+
+// ------------------------------------------------------ Jacobian pattern
+// Dependency slots of a row block: 0 own, 1..6 face neighbours (+x -x +y -y
+// +z -z), 7 the other continuum's cell.  Columns of a row are emitted in
+// increasing block offset; component-balance and total rows take every
+// column of every existing dependency, the water row the P and Sw columns of
+// non-own dependencies, fugacity and constraint rows their own block.
+__device__ __forceinline__ int dep_block(const AsmArgs& A, int k, const Geo& G, int c, int slot) {
+    if (slot == 0) return k * A.n + c;
+    if (slot == 7) return (1 - k) * A.n + c;
+    const int o = G.nb[slot - 1];
+    return o < 0 ? -1 : k * A.n + o;
+}
+
+__device__ __forceinline__ int bsize_of(const AsmArgs& A, int blk, int nc) {
+    const int k = blk / A.n, c = blk % A.n;
+    return A.lab[k][c] == 0 ? 2 * nc + 4 : nc + 3;
+}
+
+// nnz of each block's rows (for the row pointer scan)
+template <int NC>
+__global__ void k_jac_rownnz(AsmArgs A, int64_t* rownnz) {
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= 2 * A.n) return;
+    const int n = A.n, k = blk / n, c = blk % n;
+    const Geo G = geom(A, c);
+    const int lab = A.lab[k][c];
+    const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
+    const int r0 = lab == 0 ? NC : 0;
+    int full = 0, nother = 0;
+    for (int s = 0; s < 8; s++) {
+        const int d = dep_block(A, k, G, c, s);
+        if (d < 0) continue;
+        full += bsize_of(A, d, NC);
+        if (s != 0) nother++;
+    }
+    int64_t* out = rownnz + A.boff[blk];
+    if (A.phi[k * n + c] < 1e-10) {
+        for (int r = 0; r < bs; r++) out[r] = 1;   // identity row
+        return;
+    }
+    for (int r = 0; r < bs; r++) {
+        int64_t v;
+        if (r >= r0 && r < r0 + NC) v = full;
+        else if (r == r0 + NC) v = bs + 2 * nother;
+        else v = bs;
+        out[r] = v;
+    }
+}
+
+// Start of dependency block `d` within row r's column list.
+__device__ __forceinline__ int64_t seg_start(const AsmArgs& A, int k, const Geo& G, int c, int row_kind,
+                                             int d, int nc) {
+    // row_kind: 0 mass (all deps), 1 water, 2 own-only
+    int64_t pos = 0;
+    const int64_t od = A.boff[d];
+    for (int s = 0; s < 8; s++) {
+        const int e = dep_block(A, k, G, c, s);
+        if (e < 0 || e == d) continue;
+        if (A.boff[e] >= od) continue;
+        if (row_kind == 0) pos += bsize_of(A, e, nc);
+        else if (row_kind == 1) pos += (s == 0) ? bsize_of(A, e, nc) : 2;
+    }
+    return pos;
+}
+
+template <int NC, int W>
+__device__ __forceinline__ void seed_vars(const VarsT<NC, double>& v, int chunk, double pcoeff, VarsT<NC, DualV<W>>& out) {
+    out.lab = v.lab;
+    out.Po = DualV<W>(v.Po);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        out.x[i] = DualV<W>(v.x[i]);
+        out.y[i] = DualV<W>(v.y[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) out.S[j] = DualV<W>(v.S[j]);
+    // unknown order (InitGuess2, PetscAUX.cpp:892-957)
+    int col = 0;
+    const int base = chunk * W;
+    auto mark = [&](DualV<W>& t, double sc) {
+        const int rel = col - base;
+        if (rel >= 0 && rel < W) t.d[rel] = sc;
+        col++;
+    };
+    if (v.lab != -1)
+        for (int i = 0; i < NC; i++) mark(out.x[i], 1.0);
+    if (v.lab != 1)
+        for (int i = 0; i < NC; i++) mark(out.y[i], 1.0);
+    mark(out.Po, pcoeff);
+    mark(out.S[0], 1.0);
+    if (v.lab != -1) mark(out.S[1], 1.0);
+    if (v.lab != 1) mark(out.S[2], 1.0);
+}
+
+template <int NC, int W>
+__global__ __launch_bounds__(64) void k_jacobian(AsmArgs A, const int64_t* rowptr, int32_t* cols, double* vals,
+                                                 int nchunk) {
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)2 * A.n * 8 * nchunk;
+    if (item >= total) return;
+    const int chunk = (int)(item % nchunk);
+    const int slot = (int)((item / nchunk) % 8);
+    const int blk = (int)(item / (8 * nchunk));
+    const int n = A.n, k = blk / n, c = blk % n;
+    const Geo G = geom(A, c);
+    const int d = dep_block(A, k, G, c, slot);
+    if (d < 0) return;
+    const int dk = d / n, dc = d % n;
+    const int dbs = bsize_of(A, d, NC);
+    if (chunk * W >= dbs) return;
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    const int lab = v.lab;
+    const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
+    const int r0 = lab == 0 ? NC : 0;
+    using DW = DualV<W>;
+    DW rows[2 * NC + 4];
+    if (A.phi[k * n + c] < 1e-10) {
+        // zero-porosity block: identity rows (Petsc_Functions.cpp:42051-42056)
+        if (slot == 0 && chunk == 0)
+            for (int r = 0; r < bs; r++) {
+                const int64_t row = A.boff[blk] + r;
+                cols[rowptr[row]] = (int32_t)row;
+                vals[rowptr[row]] = 1.0;
+            }
+        return;
+    }
+    {
+        VarsT<NC, double> vd;
+        load_vars<NC>(A, dk, dc, vd);
+        VarsT<NC, DW> sv;
+        seed_vars<NC, W>(vd, chunk, A.pcoeff, sv);
+        double rpo = 0, rpg = 0;
+        if (dk == 0) {
+            if (A.rho_prev) {
+                rpo = A.rho_prev[dc];
+                rpg = A.rho_prev[n + dc];
+            }
+        }
+        // previous densities come from the [2n] buffer (set up by the host)
+        PropsT<NC, DW> sp;
+        propsT<NC, DW>(A, dk, dc, sv.lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp);
+        if (slot == 0) {
+            own_piece<NC, DW>(A, k, c, G, sv, sp, r0, rows);
+            for (int q = 0; q < 6; q++) {
+                const int o = G.nb[q];
+                if (o < 0) continue;
+                VarsT<NC, double> vo;
+                PropsT<NC, double> po;
+                load_vars<NC>(A, k, o, vo);
+                load_props<NC>(A, k, o, po);
+                face_piece<NC, DW, double>(A, k, G, q, c, o, sv, sp, vo, po, r0, rows);
+            }
+            VarsT<NC, double> vx;
+            PropsT<NC, double> px;
+            load_vars<NC>(A, 1 - k, c, vx);
+            load_props<NC>(A, 1 - k, c, px);
+            if (k == 0) transfer_piece<NC, DW, double>(A, 0, G, c, sv, sp, vx, px, r0, rows);
+            else transfer_piece<NC, double, DW>(A, 1, G, c, vx, px, sv, sp, r0, rows);
+        } else if (slot == 7) {
+            PropsT<NC, double> p;
+            load_props<NC>(A, k, c, p);
+            if (k == 0) transfer_piece<NC, double, DW>(A, 0, G, c, v, p, sv, sp, r0, rows);
+            else transfer_piece<NC, DW, double>(A, 1, G, c, sv, sp, v, p, r0, rows);
+        } else {
+            PropsT<NC, double> p;
+            load_props<NC>(A, k, c, p);
+            face_piece<NC, double, DW>(A, k, G, slot - 1, c, dc, v, p, sv, sp, r0, rows);
+        }
+    }
+    // scatter into CSR
+    const int pcol = (A.lab[dk][dc] == 0) ? 2 * NC : NC;
+    const int64_t rb = A.boff[blk];
+    const int64_t dboff = A.boff[d];
+    for (int r = 0; r < bs; r++) {
+        const bool mass = r >= r0 && r < r0 + NC;
+        const bool water = r == r0 + NC;
+        if (slot != 0 && !mass && !water) continue;
+        const int kind = mass ? 0 : (water ? 1 : 2);
+        const int64_t start = rowptr[rb + r] + seg_start(A, k, G, c, kind, d, NC);
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const int j = chunk * W + w;
+            if (j >= dbs) break;
+            if (water && slot != 0) {
+                if (j != pcol && j != pcol + 1) continue;
+                const int64_t pos = start + (j - pcol);
+                cols[pos] = (int32_t)(dboff + j);
+                vals[pos] = rows[r].d[w];
+            } else {
+                const int64_t pos = start + j;
+                cols[pos] = (int32_t)(dboff + j);
+                vals[pos] = rows[r].d[w];
+            }
+        }
+    }
+}
+
+// block sizes of the layout
+__global__ void k_block_sizes(const int8_t* lm, const int8_t* lf, int n, int nc, int64_t* bs) {
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= 2 * n) return;
+    const int8_t L = blk < n ? lm[blk] : lf[blk - n];
+    bs[blk] = L == 0 ? 2 * nc + 4 : nc + 3;
+}
+
+__global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
+
+}  // namespace osr
+
+// ============================================================== host side
+struct osr_asm {
+    osr_ctx* ctx = nullptr;
+    int nx = 0, ny = 0, nz = 0, n = 0, nc = 0;
+    osr::AsmConsts hK{};
+    osr::AsmConsts* dK = nullptr;
+    double *dx = nullptr, *dy = nullptr, *dz = nullptr, *h = nullptr, *phi = nullptr, *perm_m = nullptr,
+           *iftran = nullptr, *tor = nullptr;
+    double* props = nullptr;       // [NF][2n]
+    double* rho_tmp = nullptr;     // [2n]
+    int64_t* boff = nullptr;       // [2n+1]
+    int64_t* bs = nullptr;         // [2n]
+    void* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+    int64_t nrows = -1;
+};
+
+namespace {
+
+template <class Tv>
+hipError_t upload(Tv** dst, const Tv* src, size_t count) {
+    hipError_t e = hipMalloc((void**)dst, count * sizeof(Tv));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src, count * sizeof(Tv), hipMemcpyHostToDevice);
+}
+
+int exclusive_scan(osr_asm* a, const int64_t* in, int64_t* out, int64_t count, hipStream_t s) {
+    size_t need = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)count, s) != hipSuccess) return OSR_E_HIP;
+    if (need > a->scan_bytes) {
+        if (a->scan_tmp) hipFree(a->scan_tmp);
+        a->scan_tmp = nullptr;
+        OSR_HIP(hipMalloc(&a->scan_tmp, need));
+        a->scan_bytes = need;
+    }
+    OSR_HIP(hipcub::DeviceScan::ExclusiveSum(a->scan_tmp, need, in, out, (int)count, s));
+    return OSR_OK;
+}
+
+osr::AsmArgs make_args(osr_asm* a, const osr_state_dev* s) {
+    osr::AsmArgs A{};
+    A.nx = a->nx; A.ny = a->ny; A.nz = a->nz; A.n = a->n;
+    A.K = a->dK;
+    A.dx = a->dx; A.dy = a->dy; A.dz = a->dz; A.h = a->h;
+    A.phi = a->phi; A.perm_m = a->perm_m; A.iftran = a->iftran; A.tor = a->tor;
+    A.lab[0] = s->label_m; A.lab[1] = s->label_f;
+    A.Po[0] = s->Po_m; A.Po[1] = s->Po_f;
+    A.x[0] = s->x_m; A.x[1] = s->x_f;
+    A.y[0] = s->y_m; A.y[1] = s->y_f;
+    A.S[0] = s->S_m; A.S[1] = s->S_f;
+    A.rho_prev = s->rho_prev_m;
+    A.pre[0] = s->pre_m; A.pre[1] = s->pre_f;
+    A.pcoeff = s->pcoeff;
+    A.props = a->props;
+    A.boff = a->boff;
+    return A;
+}
+
+template <int NC>
+int run_props(osr_asm* a, osr::AsmArgs& A, hipStream_t st) {
+    const int nb = 2 * a->n;
+    const int grid = (nb + 255) / 256;
+    if (!A.rho_prev) {
+        // first sweep with zero previous densities, its matrix densities feed CalcIFT
+        hipLaunchKernelGGL(osr::k_props<NC>, dim3(grid), dim3(256), 0, st, A);
+        hipLaunchKernelGGL(osr::k_take_rho, dim3((a->n + 255) / 256), dim3(256), 0, st, a->props, a->n, a->rho_tmp);
+        A.rho_prev = a->rho_tmp;
+    }
+    hipLaunchKernelGGL(osr::k_props<NC>, dim3(grid), dim3(256), 0, st, A);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+int layout(osr_asm* a, const osr_state_dev* s, hipStream_t st) {
+    const int nb = 2 * a->n;
+    hipLaunchKernelGGL(osr::k_block_sizes, dim3((nb + 255) / 256), dim3(256), 0, st, s->label_m, s->label_f, a->n,
+                       a->nc, a->bs);
+    // inclusive end: scan nb+1 entries with a trailing zero
+    hipLaunchKernelGGL(osr::k_set_i64, dim3(1), dim3(1), 0, st, a->bs + nb, (int64_t)0);
+    return exclusive_scan(a, a->bs, a->boff, nb + 1, st);
+}
+
+template <int NC>
+int residual_nc(osr_asm* a, const osr_state_dev* s, double* F, hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    int rc = run_props<NC>(a, A, st);
+    if (rc) return rc;
+    const int nb = 2 * a->n;
+    hipLaunchKernelGGL(osr::k_residual<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, A, F);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+template <int NC>
+int rowptr_nc(osr_asm* a, const osr_state_dev* s, int64_t* rowptr, int64_t nrows, hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    const int nb = 2 * a->n;
+    // per-row nnz into rowptr[0..nrows), trailing 0, exclusive scan in place
+    hipLaunchKernelGGL(osr::k_jac_rownnz<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, A, rowptr);
+    hipLaunchKernelGGL(osr::k_set_i64, dim3(1), dim3(1), 0, st, rowptr + nrows, (int64_t)0);
+    return exclusive_scan(a, rowptr, rowptr, nrows + 1, st);
+}
+
+template <int NC>
+int jacobian_nc(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, int32_t* cols, double* vals,
+                hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    int rc = run_props<NC>(a, A, st);
+    if (rc) return rc;
+    constexpr int W = (2 * NC + 4) <= 16 ? (2 * NC + 4) : 8;
+    const int nchunk = (2 * NC + 4 + W - 1) / W;
+    const int64_t total = (int64_t)2 * a->n * 8 * nchunk;
+    const int64_t grid = (total + 63) / 64;
+    hipLaunchKernelGGL((osr::k_jacobian<NC, W>), dim3((unsigned)grid), dim3(64), 0, st, A, rowptr, cols, vals, nchunk);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+#define OSR_NC_SWITCH(nc, CALL)      \
+    switch (nc) {                    \
+        case 2: return CALL(2);      \
+        case 3: return CALL(3);      \
+        case 4: return CALL(4);      \
+        case 5: return CALL(5);      \
+        case 6: return CALL(6);      \
+        case 8: return CALL(8);      \
+        case 10: return CALL(10);    \
+        case 12: return CALL(12);    \
+        default: return OSR_E_NC;    \
+    }
+
+}  // namespace
+
+extern "C" {
+
+int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr_asm** out) {
+    if (!ctx || !fe || !g || !out) return OSR_E_ARG;
+    if (!fe->MW || !fe->Vc || !fe->Zc || !fe->parachor || !g->dx || !g->dy || !g->dz || !g->phi_m ||
+        !g->phi_f || !g->perm_m || !g->perm_f || !g->swt || !g->sgt)
+        return OSR_E_ARG;
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1 || g->nswt < 1 || g->nsgt < 1 || g->nswt > osr::MAXTAB ||
+        g->nsgt > osr::MAXTAB || !(g->T > 0) || !(g->dt > 0))
+        return OSR_E_ARG;
+    const int64_t n64 = (int64_t)g->nx * g->ny * g->nz;
+    if (n64 * 2 * (2 * ctx->nc + 4) > INT32_MAX) return OSR_E_ARG;   // column indices are int32
+    osr_asm* a = new (std::nothrow) osr_asm();
+    if (!a) return OSR_E_ARG;
+    a->ctx = ctx;
+    a->nx = g->nx; a->ny = g->ny; a->nz = g->nz; a->n = (int)n64; a->nc = ctx->nc;
+    const int n = a->n, nc = a->nc;
+    osr::AsmConsts& K = a->hK;
+    K.nc = nc;
+    K.eos = ctx->eos;
+    K.T = g->T;
+    const osr::FluidConsts<double>& F = ctx->host_fluid.d;
+    K.d1 = F.d1;
+    K.d2 = F.d2;
+    for (int i = 0; i < nc; i++) {
+        const double sa = F.sac[i] * (1 + F.m[i] * (1 - std::sqrt(g->T / F.Tc[i])));
+        K.sa[i] = sa;
+        K.a[i] = sa * sa;
+        K.b[i] = F.b[i];
+        K.tc[i] = F.Tc[i];
+        K.pc[i] = F.Pc[i];
+        K.mw[i] = fe->MW[i];
+        K.sqmw[i] = std::sqrt(fe->MW[i]);
+        K.vcm3[i] = fe->Vc[i] / 1000000.0;
+        K.v53[i] = std::pow(K.vcm3[i], 5.0 / 3.0);
+        K.v23[i] = std::pow(K.vcm3[i], 2.0 / 3.0);
+        K.par[i] = fe->parachor[i];
+        const double tr = g->T / F.Tc[i];
+        const double gam = std::pow(F.Tc[i], 1.0 / 6) / (std::sqrt(fe->MW[i]) * std::pow(F.Pc[i] / 101325, 2.0 / 3));
+        K.mu0[i] = (4.61 * std::pow(tr, 0.618) - 2.04 * std::exp(-0.449 * tr) + 1.94 * std::exp(-4.058 * tr) + 0.1) *
+                   1e-4 / gam;
+        for (int j = 0; j < nc; j++) K.bic[i * nc + j] = F.bic[i * nc + j];
+    }
+    // binary diffusion table (Mass_Tran.cpp:41-45), T-only
+    double coldia[osr::NCMAX], epdia[osr::NCMAX];
+    for (int i = 0; i < nc; i++) {
+        coldia[i] = 0.1866 * std::pow(fe->Vc[i], 1.0 / 3.0) * std::pow(fe->Zc[i], -1.2);
+        epdia[i] = 65.3 * F.Tc[i] * std::pow(fe->Zc[i], 3.6);
+    }
+    for (int j = 0; j < nc; j++) {
+        K.pdlast[j] = 0;
+        for (int k = 0; k < nc; k++) {
+            if (j == k) { K.pdij[j * nc + k] = 0; continue; }
+            const double Sij = (coldia[j] + coldia[k]) / 2;
+            const double Tij = g->T / std::sqrt(epdia[j] * epdia[k]);
+            const double Oij = 1.06306 * std::pow(Tij, -0.1561) + 0.193 * std::exp(-0.47635 * Tij) +
+                               1.03587 * std::exp(-1.52996 * Tij) + 1.76474 * std::exp(-3.89411 * Tij);
+            K.pdij[j * nc + k] = 188.2922475 * std::sqrt(g->T * (1.0 / fe->MW[j] + 1.0 / fe->MW[k])) /
+                                 (osr::RGAS * Oij * Sij * Sij);
+            K.pdlast[j] = K.pdij[j * nc + k];
+        }
+    }
+    for (int r = 0; r < 4 * g->nswt; r++) K.swt[r] = g->swt[r];
+    for (int r = 0; r < 4 * g->nsgt; r++) K.sgt[r] = g->sgt[r];
+    K.nswt = g->nswt;
+    K.nsgt = g->nsgt;
+    K.refP = g->refP; K.cpor = g->cpor; K.dcpor = g->dcpor; K.dt = g->dt;
+    K.wat_ro = g->wat_ro; K.wat_mu = g->wat_mu; K.sigma = g->sigma; K.lf = g->lf;
+    K.cdiff = g->cdiff; K.const_diff = g->const_diff;
+    // host geometry: block heights (CalcBlockHeight) and face kA/h (IFK_Calc)
+    std::vector<double> hh(g->nz), phi(2 * (size_t)n), ift(6 * (size_t)n, 0.0), tor(n, 1.0), permm(3 * (size_t)n);
+    double acc = 0;
+    for (int k = g->nz - 1; k >= 0; k--) {
+        if (k == g->nz - 1) acc = 0.5 * g->dz[g->nz - 1];
+        else acc += 0.5 * (g->dz[k] + g->dz[k + 1]);
+        hh[k] = acc;
+    }
+    for (int c = 0; c < n; c++) {
+        phi[c] = g->phi_m[c];
+        phi[n + c] = g->phi_f[c];
+        if (g->tor) tor[c] = g->tor[c];
+    }
+    for (int i = 0; i < 3 * n; i++) permm[i] = g->perm_m[i];
+    for (int iz = 0; iz < g->nz; iz++)
+        for (int iy = 0; iy < g->ny; iy++)
+            for (int ix = 0; ix < g->nx; ix++) {
+                const int c = iz * g->nx * g->ny + iy * g->nx + ix;
+                for (int k = 0; k < 2; k++) {
+                    const double* ph = k == 0 ? g->phi_m : g->phi_f;
+                    const double* pm = k == 0 ? g->perm_m : g->perm_f;
+                    double* o = ift.data() + (size_t)k * 3 * n;
+                    if (g->nx != 1 && ix != g->nx - 1 && ph[c] > 1e-10 && ph[c + 1] > 1e-10)
+                        o[c] = osr::SI_C * 2 * g->dy[iy] * g->dz[iz] / (g->dx[ix] / pm[c] + g->dx[ix + 1] / pm[c + 1]);
+                    const int cy = c + g->nx;
+                    if (g->ny != 1 && iy != g->ny - 1 && ph[c] > 1e-10 && ph[cy] > 1e-10)
+                        o[n + c] = osr::SI_C * 2 * g->dx[ix] * g->dz[iz] /
+                                   (g->dy[iy] / pm[n + c] + g->dy[iy + 1] / pm[n + cy]);
+                    const int cz = c + g->nx * g->ny;
+                    if (g->nz != 1 && iz != g->nz - 1 && ph[c] > 1e-10 && ph[cz] > 1e-10)
+                        o[2 * n + c] = osr::SI_C * 2 * g->dx[ix] * g->dy[iy] /
+                                       (g->dz[iz] / pm[2 * n + c] + g->dz[iz + 1] / pm[2 * n + cz]);
+                }
+            }
+    const int nf = 16 + 2 * nc;
+    bool ok = upload(&a->dK, &a->hK, 1) == hipSuccess && upload(&a->dx, g->dx, g->nx) == hipSuccess &&
+              upload(&a->dy, g->dy, g->ny) == hipSuccess && upload(&a->dz, g->dz, g->nz) == hipSuccess &&
+              upload(&a->h, hh.data(), hh.size()) == hipSuccess && upload(&a->phi, phi.data(), phi.size()) == hipSuccess &&
+              upload(&a->perm_m, permm.data(), permm.size()) == hipSuccess &&
+              upload(&a->iftran, ift.data(), ift.size()) == hipSuccess &&
+              upload(&a->tor, tor.data(), tor.size()) == hipSuccess &&
+              hipMalloc((void**)&a->props, (size_t)nf * 2 * n * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&a->rho_tmp, 2 * (size_t)n * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&a->boff, (2 * (size_t)n + 1) * sizeof(int64_t)) == hipSuccess &&
+              hipMalloc((void**)&a->bs, (2 * (size_t)n + 1) * sizeof(int64_t)) == hipSuccess;
+    if (!ok) {
+        osr_asm_destroy(a);
+        return OSR_E_HIP;
+    }
+    *out = a;
+    return OSR_OK;
+}
+
+int osr_asm_destroy(osr_asm* a) {
+    if (!a) return OSR_E_ARG;
+    void* ptrs[] = {a->dK, a->dx, a->dy, a->dz, a->h, a->phi, a->perm_m, a->iftran, a->tor,
+                    a->props, a->rho_tmp, a->boff, a->bs, a->scan_tmp};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete a;
+    return OSR_OK;
+}
+
+int osr_asm_layout(osr_asm* a, const osr_state_dev* s, int64_t* offsets, int64_t* nrows, void* stream) {
+    if (!a || !s || !s->label_m || !s->label_f) return OSR_E_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = layout(a, s, st);
+    if (rc) return rc;
+    int64_t total = 0;
+    OSR_HIP(hipMemcpyAsync(&total, a->boff + 2 * a->n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    OSR_HIP(hipStreamSynchronize(st));
+    a->nrows = total;
+    if (nrows) *nrows = total;
+    if (offsets)
+        OSR_HIP(hipMemcpyAsync(offsets, a->boff, (2 * (size_t)a->n + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+    return OSR_OK;
+}
+
+static int check_state(osr_asm* a, const osr_state_dev* s) {
+    if (!a || !s || !s->label_m || !s->label_f || !s->Po_m || !s->Po_f || !s->x_m || !s->y_m || !s->x_f ||
+        !s->y_f || !s->S_m || !s->S_f || !(s->pcoeff > 0))
+        return OSR_E_ARG;
+    if (a->nrows < 0) return OSR_E_ARG;   // osr_asm_layout first
+    return OSR_OK;
+}
+
+int osr_residual_device(osr_asm* a, const osr_state_dev* s, double* F, void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!F) return OSR_E_ARG;
+#define CALL(NC) residual_nc<NC>(a, s, F, (hipStream_t)stream)
+    OSR_NC_SWITCH(a->nc, CALL)
+#undef CALL
+}
+
+int osr_jacobian_rowptr_device(osr_asm* a, const osr_state_dev* s, int64_t* rowptr, int64_t* nnz, void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!rowptr) return OSR_E_ARG;
+    hipStream_t st = (hipStream_t)stream;
+#define CALL(NC) rowptr_nc<NC>(a, s, rowptr, a->nrows, st)
+    switch (a->nc) {
+        case 2: rc = CALL(2); break;
+        case 3: rc = CALL(3); break;
+        case 4: rc = CALL(4); break;
+        case 5: rc = CALL(5); break;
+        case 6: rc = CALL(6); break;
+        case 8: rc = CALL(8); break;
+        case 10: rc = CALL(10); break;
+        case 12: rc = CALL(12); break;
+        default: return OSR_E_NC;
+    }
+#undef CALL
+    if (rc) return rc;
+    if (nnz) {
+        OSR_HIP(hipMemcpyAsync(nnz, rowptr + a->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        OSR_HIP(hipStreamSynchronize(st));
+    }
+    return OSR_OK;
+}
+
+int osr_jacobian_device(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, int32_t* cols, double* vals,
+                        void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!rowptr || !cols || !vals) return OSR_E_ARG;
+#define CALL(NC) jacobian_nc<NC>(a, s, rowptr, cols, vals, (hipStream_t)stream)
+    OSR_NC_SWITCH(a->nc, CALL)
+#undef CALL
+}
+
+int osr_props_device(osr_asm* a, const osr_state_dev* s, double* out, void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!out) return OSR_E_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    osr::AsmArgs A = make_args(a, s);
+#define CALL(NC) run_props<NC>(a, A, st)
+    switch (a->nc) {
+        case 2: rc = CALL(2); break;
+        case 3: rc = CALL(3); break;
+        case 4: rc = CALL(4); break;
+        case 5: rc = CALL(5); break;
+        case 6: rc = CALL(6); break;
+        case 8: rc = CALL(8); break;
+        case 10: rc = CALL(10); break;
+        case 12: rc = CALL(12); break;
+        default: return OSR_E_NC;
+    }
+#undef CALL
+    if (rc) return rc;
+    const size_t nf = 16 + 2 * (size_t)a->nc;
+    OSR_HIP(hipMemcpyAsync(out, a->props, nf * 2 * a->n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    return OSR_OK;
+}
+
+}  // extern "C"

@@ -103,4 +103,10 @@ int osr_psat_device(osr_ctx* ctx, int64_t n, const double* T, double T0, const d
     return osr::launch_psat(ctx, n, T, T0, P, kind, x, y, psat, xm, (hipStream_t)stream);
 }
 
+int osr_phase_ctrl_device(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* label,
+                          const double* Po, const double* Pg, double* x, double* y, double* S,
+                          double* psat, void* stream) {
+    return osr::phase_ctrl(ctx, n, T, T0, label, Po, Pg, x, y, S, psat, (hipStream_t)stream);
+}
+
 }  // extern "C"

@@ -264,10 +264,11 @@ bool nc_supported(int nc) {
 int launch_flash(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
                  const double* z, double sw, const osr_flash_out* out, int mode,
                  hipStream_t stream) {
-    if (!ctx || n < 0 || !P || !z || !out) return OSR_E_ARG;
+    if (!ctx || n < 0 || !out) return OSR_E_ARG;
     if (mode != OSR_FLASH_PLAIN && mode != OSR_FLASH_PERSISTENT && mode != OSR_FLASH_MIXED)
         return OSR_E_ARG;
     if (n == 0) return OSR_OK;
+    if (!P || !z || (!T && !(T0 > 0))) return OSR_E_ARG;
     FlashArgs a;
     a.n = n;
     a.T = T;

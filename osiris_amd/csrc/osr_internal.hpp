@@ -42,5 +42,7 @@ int launch_flash(osr_ctx* ctx, int64_t n, const double* T, double T0, const doub
 int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
                 const int8_t* kind, const double* x, const double* y, double* psat,
                 double* xm, hipStream_t stream);
+int phase_ctrl(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* label, const double* Po,
+               const double* Pg, double* x, double* y, double* S, double* psat_out, hipStream_t st);
 bool nc_supported(int nc);
 }  // namespace osr

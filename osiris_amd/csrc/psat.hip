@@ -354,9 +354,10 @@ static int launch_psat_nc(osr_ctx* ctx, const PsatArgs& a, hipStream_t stream) {
 int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
                 const int8_t* kind, const double* x, const double* y, double* psat,
                 double* xm, hipStream_t stream) {
-    if (!ctx || n < 0 || !P || !kind || !x || !y || !psat) return OSR_E_ARG;
-    if (!T && !(T0 > 0)) return OSR_E_ARG;
+    if (!ctx || n < 0) return OSR_E_ARG;
     if (n == 0) return OSR_OK;
+    if (!P || !kind || !x || !y || !psat) return OSR_E_ARG;
+    if (!T && !(T0 > 0)) return OSR_E_ARG;
     PsatArgs a{n, T, T0, P, kind, x, y, psat, xm};
     switch (ctx->nc) {
         case 2: return launch_psat_nc<2>(ctx, a, stream);
@@ -369,6 +370,100 @@ int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const doubl
         case 12: return launch_psat_nc<12>(ctx, a, stream);
         default: return OSR_E_NC;
     }
+}
+
+// PhaseCtrl / fPhaseCtrl (Do_Cycle.cpp:1558-1722), split in three launches:
+// k_phase_kind picks the saturation test per cell (dew at the gas pressure
+// for gas cells, bubble at the oil pressure for liquid cells), k_psat runs
+// it, k_phase_apply applies the label/saturation/composition rules.
+constexpr double PHASE_ENV = 0.95;   // PAHSE_ENV_BOUNDARY, Functions.h:11
+constexpr double EPS_SAT = 1e-3;     // Functions.h:226
+constexpr double EPS_SAT_FRAC = 0;   // Functions.h:12
+
+__global__ void k_phase_kind(int64_t n, const int8_t* label, const double* Po, const double* Pg,
+                             int8_t* kind, double* Pin) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int8_t L = label[c];
+    kind[c] = (L == 1 || L == -1) ? L : 0;
+    Pin[c] = L == -1 ? Pg[c] : Po[c];
+}
+
+template <int NC>
+__global__ void k_phase_apply(int64_t n, int8_t* label, const double* Po, const double* Pg, double* x,
+                              double* y, double* S, const double* psat, const double* xm) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int L = label[c];
+    double Sw = S[c], So = S[n + c], Sg = S[2 * n + c];
+    const double ps = psat[c];
+    bool done = false;
+    if (L == -1 && Pg[c] <= PHASE_ENV * ps) {
+        So = EPS_SAT;
+        Sg -= EPS_SAT;
+        L = 0;
+#pragma unroll
+        for (int i = 0; i < NC; i++) x[(int64_t)i * n + c] = xm[(int64_t)i * n + c];
+        done = true;
+    } else if (L == 1 && Po[c] <= PHASE_ENV * ps) {
+        Sg = EPS_SAT;
+        So -= EPS_SAT;
+        L = 0;
+#pragma unroll
+        for (int i = 0; i < NC; i++) y[(int64_t)i * n + c] = xm[(int64_t)i * n + c];
+        done = true;
+    }
+    if (!done) {
+        if (So <= EPS_SAT * EPS_SAT_FRAC) {
+            So = 0;
+            Sg = 1 - Sw;
+            L = -1;
+        }
+        if (Sg <= EPS_SAT * EPS_SAT_FRAC) {
+            Sg = 0;
+            So = 1 - Sw;
+            L = 1;
+        }
+    }
+    const double inv = 1.0 / (Sw + So + Sg);
+    S[c] = Sw * inv;
+    S[n + c] = So * inv;
+    S[2 * n + c] = Sg * inv;
+    label[c] = (int8_t)L;
+}
+
+int phase_ctrl(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* label, const double* Po,
+               const double* Pg, double* x, double* y, double* S, double* psat_out, hipStream_t st) {
+    if (!ctx || n < 0) return OSR_E_ARG;
+    if (n == 0) return OSR_OK;
+    if (!label || !Po || !Pg || !x || !y || !S) return OSR_E_ARG;
+    if (!T && !(T0 > 0)) return OSR_E_ARG;
+    const size_t nc = ctx->nc;
+    const size_t need = n * (sizeof(int8_t) + 2 * sizeof(double) + nc * sizeof(double)) + 1024;
+    if (need > ctx->queue_bytes) {
+        if (ctx->d_queue) hipFree(ctx->d_queue);
+        ctx->d_queue = nullptr;
+        ctx->queue_bytes = 0;
+        OSR_HIP(hipMalloc(&ctx->d_queue, need));
+        ctx->queue_bytes = need;
+    }
+    char* base = (char*)ctx->d_queue;
+    double* ps = (double*)base;
+    double* Pin = ps + n;
+    double* xm = Pin + n;
+    int8_t* kind = (int8_t*)(xm + nc * n);
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_phase_kind, dim3(g), dim3(256), 0, st, n, label, Po, Pg, kind, Pin);
+    int rc = launch_psat(ctx, n, T, T0, Pin, kind, x, y, ps, xm, st);
+    if (rc) return rc;
+    switch (ctx->nc) {
+#define APPLY(NC) case NC: hipLaunchKernelGGL(k_phase_apply<NC>, dim3(g), dim3(256), 0, st, n, label, Po, Pg, x, y, S, ps, xm); break;
+        APPLY(2) APPLY(3) APPLY(4) APPLY(5) APPLY(6) APPLY(8) APPLY(10) APPLY(12)
+#undef APPLY
+        default: return OSR_E_NC;
+    }
+    if (psat_out) OSR_HIP(hipMemcpyAsync(psat_out, ps, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
 }  // namespace osr

@@ -175,3 +175,172 @@ def psat_numpy(ctx):
         ps, xm = psat(ctx, T, P, kind, x, y)
         return ps.cpu().numpy(), xm.cpu().numpy()
     return fn
+
+
+# ------------------------------------------------------------------ assembly
+class _FluidExt(C.Structure):
+    _fields_ = [("MW", _dp), ("Vc", _dp), ("Zc", _dp), ("parachor", _dp)]
+
+
+class _Grid(C.Structure):
+    _fields_ = [("dx", _dp), ("dy", _dp), ("dz", _dp), ("phi_m", _dp), ("phi_f", _dp),
+                ("perm_m", _dp), ("perm_f", _dp), ("tor", _dp), ("swt", _dp), ("sgt", _dp),
+                ("refP", C.c_double), ("cpor", C.c_double), ("dcpor", C.c_double),
+                ("sigma", C.c_double), ("lf", C.c_double), ("cdiff", C.c_double),
+                ("T", C.c_double), ("dt", C.c_double), ("wat_ro", C.c_double),
+                ("wat_mu", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("const_diff", C.c_int32), ("nswt", C.c_int32), ("nsgt", C.c_int32)]
+
+
+class _StateDev(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in
+                ("label_m", "label_f", "Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f", "S_m", "S_f",
+                 "rho_prev_m", "pre_m", "pre_f")] + [("pcoeff", C.c_double)]
+
+
+def _bind_asm(L):
+    if getattr(L, "_asm_bound", False):
+        return
+    L.osr_asm_create.argtypes = [C.c_void_p, C.POINTER(_FluidExt), C.POINTER(_Grid),
+                                 C.POINTER(C.c_void_p)]
+    L.osr_asm_destroy.argtypes = [C.c_void_p]
+    L.osr_asm_layout.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p,
+                                 C.POINTER(C.c_int64), C.c_void_p]
+    L.osr_residual_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
+    L.osr_jacobian_rowptr_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p,
+                                             C.POINTER(C.c_int64), C.c_void_p]
+    L.osr_jacobian_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
+    L.osr_props_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
+    L._asm_bound = True
+
+
+class Assembler:
+    """Residual/Jacobian assembly for one grid (an osr_asm) -- the role of
+    FormFunction/FormJacobian (compsim/PetscAUX.cpp:59-173)."""
+
+    def __init__(self, ctx, fluid, grid):
+        L = lib()
+        _bind_asm(L)
+        self.ctx = ctx
+        self.nc = ctx.nc
+        self.n = grid.nx * grid.ny * grid.nz
+        keep = []
+
+        def P(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data_as(_dp)
+
+        fe = _FluidExt(P(fluid.MW), P(fluid.Vc), P(fluid.Zc), P(fluid.parachor))
+        g = _Grid(P(grid.dx), P(grid.dy), P(grid.dz), P(grid.phi_m), P(grid.phi_f),
+                  P(grid.perm_m), P(grid.perm_f), P(grid.tor), P(grid.swt), P(grid.sgt),
+                  grid.refP, grid.cpor, grid.dcpor, grid.sigma, grid.lf, grid.cdiff, grid.T,
+                  grid.dt, grid.wat_ro, grid.wat_mu, grid.nx, grid.ny, grid.nz,
+                  int(grid.const_diff), len(grid.swt), len(grid.sgt))
+        h = C.c_void_p()
+        _check(L.osr_asm_create(ctx.h, C.byref(fe), C.byref(g), C.byref(h)), "osr_asm_create")
+        self.h = h
+        self.nrows = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().osr_asm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_state(self, st):
+        """Upload a host state (synth.State-like) to device tensors."""
+        import torch
+        d = {}
+        for k in ("label_m", "label_f"):
+            d[k] = torch.as_tensor(np.ascontiguousarray(getattr(st, k), np.int8)).cuda()
+        for k in ("Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f", "S_m", "S_f", "rho_prev_m",
+                  "pre_m", "pre_f"):
+            v = getattr(st, k, None)
+            d[k] = None if v is None else torch.as_tensor(
+                np.ascontiguousarray(v, np.float64)).cuda()
+        d["pcoeff"] = float(st.pcoeff)
+        return d
+
+    def _sd(self, d):
+        return _StateDev(*[(d[k].data_ptr() if d.get(k) is not None else None)
+                           for k, _ in _StateDev._fields_[:-1]], d["pcoeff"])
+
+    def layout(self, d, stream=None):
+        import torch
+        off = torch.empty(2 * self.n + 1, dtype=torch.int64, device="cuda")
+        nrows = C.c_int64()
+        sd = self._sd(d)
+        _check(lib().osr_asm_layout(self.h, C.byref(sd), C.c_void_p(off.data_ptr()),
+                                    C.byref(nrows), _stream_ptr(stream)), "osr_asm_layout")
+        self.nrows = nrows.value
+        return off, nrows.value
+
+    def residual(self, d, out=None, stream=None):
+        import torch
+        if self.nrows is None:
+            self.layout(d, stream)
+        F = out if out is not None else torch.empty(self.nrows, dtype=torch.float64, device="cuda")
+        sd = self._sd(d)
+        _check(lib().osr_residual_device(self.h, C.byref(sd), C.c_void_p(F.data_ptr()),
+                                         _stream_ptr(stream)), "osr_residual_device")
+        return F
+
+    def jacobian(self, d, stream=None):
+        """Returns (rowptr, cols, vals) as device tensors."""
+        import torch
+        if self.nrows is None:
+            self.layout(d, stream)
+        rowptr = torch.empty(self.nrows + 1, dtype=torch.int64, device="cuda")
+        nnz = C.c_int64()
+        sd = self._sd(d)
+        _check(lib().osr_jacobian_rowptr_device(self.h, C.byref(sd), C.c_void_p(rowptr.data_ptr()),
+                                                C.byref(nnz), _stream_ptr(stream)),
+               "osr_jacobian_rowptr_device")
+        cols = torch.empty(nnz.value, dtype=torch.int32, device="cuda")
+        vals = torch.empty(nnz.value, dtype=torch.float64, device="cuda")
+        _check(lib().osr_jacobian_device(self.h, C.byref(sd), C.c_void_p(rowptr.data_ptr()),
+                                         C.c_void_p(cols.data_ptr()), C.c_void_p(vals.data_ptr()),
+                                         _stream_ptr(stream)), "osr_jacobian_device")
+        return rowptr, cols, vals
+
+    def props(self, d, stream=None):
+        import torch
+        nf = 16 + 2 * self.nc
+        out = torch.empty(nf, 2 * self.n, dtype=torch.float64, device="cuda")
+        sd = self._sd(d)
+        _check(lib().osr_props_device(self.h, C.byref(sd), C.c_void_p(out.data_ptr()),
+                                      _stream_ptr(stream)), "osr_props_device")
+        return out
+
+
+def phase_ctrl(ctx, T, label, Po, Pg, x, y, S, stream=None):
+    """PhaseCtrl (compsim/Do_Cycle.cpp:1558-1722) on one continuum.  label,
+    x, y, S are device tensors updated in place; returns psat."""
+    import torch
+    L = lib()
+    if not getattr(L, "_pc_bound", False):
+        L.osr_phase_ctrl_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_double,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L._pc_bound = True
+    n = label.shape[0]
+    Po = _as_dev(Po, torch.float64)
+    Pg = _as_dev(Pg, torch.float64)
+    if np.isscalar(T):
+        Tt, T0 = None, float(T)
+    else:
+        Tt, T0 = _as_dev(T, torch.float64), 0.0
+    ps = torch.empty(n, dtype=torch.float64, device="cuda")
+    _check(L.osr_phase_ctrl_device(ctx.h, n, _ptr(Tt), T0, _ptr(label), _ptr(Po), _ptr(Pg),
+                                   _ptr(x), _ptr(y), _ptr(S), _ptr(ps), _stream_ptr(stream)),
+           "osr_phase_ctrl_device")
+    return ps
