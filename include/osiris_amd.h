@@ -205,6 +205,13 @@ int osr_jacobian_device(osr_asm *a, const osr_state_dev *s, const int64_t *rowpt
  * D_o[nc], D_g[nc]. */
 int osr_props_device(osr_asm *a, const osr_state_dev *s, double *out, void *stream);
 
+/* Launch-shape knobs (no counterpart in the reference; set by the
+ * autotuner, osiris_amd/tune.py): "flash.refill_min" (1..64: idle lanes a
+ * wave waits for before refilling from the work queue) and
+ * "flash.blocks_per_cu" (0 = occupancy-derived; else 256-thread blocks per
+ * CU of the persistent flash grid). */
+int osr_tune_set(osr_ctx *ctx, const char *key, double value);
+
 /* Library version string and the component counts compiled in. */
 const char *osr_version(void);
 int osr_supported_nc(int32_t nc);

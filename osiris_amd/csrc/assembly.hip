@@ -170,11 +170,12 @@ __device__ __forceinline__ void table(const double* tb, int nr, const T& s, T& k
         pc = T(tb[4 * (nr - 1) + 3]);
         kro = T(tb[4 * (nr - 1) + 2]);
     } else {
+        // slope as a quotient, as the reference writes it (Do_Cycle.cpp:183-185),
+        // so a saturation on a table node gives the node value exactly
         const double s0 = tb[4 * (r - 1)], s1 = tb[4 * r];
-        const double inv = 1.0 / (s1 - s0);
-        kr = (tb[4 * r + 1] - tb[4 * (r - 1) + 1]) * inv * (s - s1) + tb[4 * r + 1];
-        kro = (tb[4 * r + 2] - tb[4 * (r - 1) + 2]) * inv * (s - s1) + tb[4 * r + 2];
-        pc = (tb[4 * r + 3] - tb[4 * (r - 1) + 3]) * inv * (s - s1) + tb[4 * r + 3];
+        kr = (tb[4 * r + 1] - tb[4 * (r - 1) + 1]) / (s1 - s0) * (s - s1) + tb[4 * r + 1];
+        kro = (tb[4 * r + 2] - tb[4 * (r - 1) + 2]) / (s1 - s0) * (s - s1) + tb[4 * r + 2];
+        pc = (tb[4 * r + 3] - tb[4 * (r - 1) + 3]) / (s1 - s0) * (s - s1) + tb[4 * r + 3];
     }
 }
 

@@ -1,6 +1,7 @@
 // capi.hip -- the extern "C" entry points of libosiris_amd.so.
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <new>
 
 #include "osr_internal.hpp"
@@ -101,6 +102,21 @@ int osr_psat_device(osr_ctx* ctx, int64_t n, const double* T, double T0, const d
                     const int8_t* kind, const double* x, const double* y, double* psat,
                     double* xm, void* stream) {
     return osr::launch_psat(ctx, n, T, T0, P, kind, x, y, psat, xm, (hipStream_t)stream);
+}
+
+int osr_tune_set(osr_ctx* ctx, const char* key, double value) {
+    if (!ctx || !key) return OSR_E_ARG;
+    const std::string k(key);
+    if (k == "flash.refill_min") {
+        if (value < 1 || value > 64) return OSR_E_ARG;
+        ctx->refill_min = (int)value;
+    } else if (k == "flash.blocks_per_cu") {
+        if (value < 0 || value > 64) return OSR_E_ARG;
+        ctx->flash_blocks_per_cu = (int)value;
+    } else {
+        return OSR_E_ARG;
+    }
+    return OSR_OK;
 }
 
 int osr_phase_ctrl_device(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* label,

@@ -70,14 +70,17 @@ __device__ __forceinline__ bool m_isnan(float v) { return v != v; }
 // reference's, written out literally (not an assumed ordering).
 template <class R>
 __device__ __forceinline__ R solve_z(R a1, R a2, R a3, bool gas) {
-    const R Q = (R(3) * a2 - a1 * a1) / R(9);
-    const R J = (R(9) * a1 * a2 - R(27) * a3 - R(2) * a1 * a1 * a1) / R(54);
+    // divisions by the constants 9, 54 and 3 as products with their
+    // reciprocals (one rounding each instead of a division sequence)
+    const R third = R(0.33333333333333333333333333333333);
+    const R Q = (R(3) * a2 - a1 * a1) * R(0.11111111111111111111111111111111);
+    const R J = (R(9) * a1 * a2 - R(27) * a3 - R(2) * a1 * a1 * a1) * R(0.018518518518518518518518518518519);
     const R D = Q * Q * Q + J * J;
     R Z;
     if (D < 0) {
         const R r = R(2) * m_sqrt(-Q);
-        const R s = a1 / R(3);
-        const R t = m_acos(J / m_sqrt(-Q * Q * Q)) / R(3);
+        const R s = a1 * third;
+        const R t = m_acos(J / m_sqrt(-Q * Q * Q)) * third;
         R sn, cs;
         m_sincos(t, &sn, &cs);
         const R h = R(0.86602540378443864676372317075294);
@@ -95,13 +98,13 @@ __device__ __forceinline__ R solve_z(R a1, R a2, R a3, bool gas) {
         }
     } else if (D == 0) {
         const R t = m_cbrt(J);
-        const R x1 = R(2) * t - a1 / R(3);
-        const R x2 = -t - a1 / R(3);
+        const R x1 = R(2) * t - a1 * third;
+        const R x2 = -t - a1 * third;
         if (gas) Z = (x2 >= x1) ? x2 : x1;
         else Z = (x2 <= x1 && x2 > 0) ? x2 : x1;
     } else {
         const R sd = m_sqrt(D);
-        Z = m_cbrt(J + sd) + m_cbrt(J - sd) - a1 / R(3);
+        Z = m_cbrt(J + sd) + m_cbrt(J - sd) - a1 * third;
     }
     return Z;
 }
@@ -142,15 +145,15 @@ struct PhaseInv {
     R Z, A, B, invCa, invCb, lnZB, DDFF;
 };
 
+// pA = P/(R T)^2, pB = P/(R T): per-cell constants of the flash.
 template <int NC, class R>
 __device__ __forceinline__ PhaseInv<R> phase_eval(const FluidConsts<R>& C, bool srk, const R (&sa)[NC],
-                                                  const R (&x)[NC], R P, R T, bool gas, R (&t)[NC]) {
+                                                  const R (&x)[NC], R pA, R pB, bool gas, R (&t)[NC]) {
     R Ca, Cb;
     mix<NC, R>(C, sa, x, t, Ca, Cb);
     PhaseInv<R> p;
-    const R rt = R(RGAS) * T;
-    p.A = Ca * P / (rt * rt);
-    p.B = Cb * P / rt;
+    p.A = Ca * pA;
+    p.B = Cb * pB;
     p.Z = phase_z<R>(srk, p.A, p.B, gas);
     p.invCa = R(1) / Ca;
     p.invCb = R(1) / Cb;

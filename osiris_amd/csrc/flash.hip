@@ -40,6 +40,7 @@ struct SSCell {
     R z[NC];
     R sa[NC];
     R P, T, L, Lb, Zl, Zg;
+    R pA, pB;   // P/(R T)^2, P/(R T)
     int it, rr;
 };
 
@@ -87,8 +88,8 @@ __device__ __forceinline__ R ss_pass(const FluidConsts<R>& C, bool srk, SSCell<N
 
     // Both phases at the oil pressure (:660-664).
     R tl[NC], tg[NC];
-    const PhaseInv<R> pl = phase_eval<NC, R>(C, srk, s.sa, x, s.P, s.T, false, tl);
-    const PhaseInv<R> pg = phase_eval<NC, R>(C, srk, s.sa, y, s.P, s.T, true, tg);
+    const PhaseInv<R> pl = phase_eval<NC, R>(C, srk, s.sa, x, s.pA, s.pB, false, tl);
+    const PhaseInv<R> pg = phase_eval<NC, R>(C, srk, s.sa, y, s.pA, s.pB, true, tg);
     s.Zl = pl.Z;
     s.Zg = pg.Z;
 
@@ -127,6 +128,9 @@ __device__ __forceinline__ void cell_init(const FluidConsts<R>& C, const FlashAr
     for (int i = 0; i < NC; i++) s.z[i] = R(a.z[(int64_t)i * a.n + c]);
     sqrt_a<NC, R>(C, s.T, s.sa);
     wilson<NC, R>(C, s.T, s.P, s.K);
+    const R rt = R(RGAS) * s.T;
+    s.pB = s.P / rt;
+    s.pA = s.pB / rt;
     s.L = 0;
     s.Lb = R(10);
     s.Zl = s.Zg = 0;
@@ -243,7 +247,7 @@ static int launch_nc(osr_ctx* ctx, const FlashArgs& a, int mode, hipStream_t str
     int per_cu = 0;
     OSR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_flash_persistent<NC>, block, 0));
     if (per_cu < 1) per_cu = 1;
-    if (ctx->flash_waves_per_simd > 0) per_cu = ctx->flash_waves_per_simd;  // 4 waves per block = 1 per SIMD
+    if (ctx->flash_blocks_per_cu > 0) per_cu = ctx->flash_blocks_per_cu;  // 4 waves per block = 1 per SIMD
     int64_t grid = (int64_t)per_cu * ctx->num_cu;
     const int64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;

@@ -22,7 +22,7 @@ struct osr_ctx {
     size_t queue_bytes = 0;
     // Tuning knobs (set by osr_tune_set; defaults chosen from measurement).
     int flash_block = 256;
-    int flash_waves_per_simd = 0;   // 0 = from occupancy query
+    int flash_blocks_per_cu = 0;   // 0 = from occupancy query
     int refill_min = 16;
     int mixed_warm_iters = 2;
     float mixed_tol32 = 1e-5f;

@@ -99,6 +99,11 @@ class Context:
         except Exception:
             pass
 
+    def tune(self, key, value):
+        L = lib()
+        L.osr_tune_set.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        _check(L.osr_tune_set(self.h, key.encode(), float(value)), f"osr_tune_set({key})")
+
 
 def _as_dev(t, dtype=None):
     import torch

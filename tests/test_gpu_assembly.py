@@ -70,7 +70,9 @@ def test_assembly_parity(gpu, oracle):
             F = oracle.residual(fl, grid, s)
             Fs = oracle.residual_scale(fl, grid, s)
             assert np.array_equal(np.isfinite(F), np.isfinite(g["F"])), name
-            err = np.abs(g["F"] - F) / np.maximum(Fs, 1e-300)
+            # rows whose every term is zero (e.g. water rows at a zero-kr
+            # saturation) are held to 1e-14 of the largest row scale
+            err = np.abs(g["F"] - F) / np.maximum(Fs, 1e-14 * np.nanmax(Fs) + 1e-300)
             assert err[np.isfinite(F)].max() <= RTOL, f"{name}/{label}: residual err {err.max():.3e}"
             rp, cols, vals = oracle.jacobian(fl, grid, s)
             assert np.array_equal(g["rp"], rp), f"{name}: row pointer"
