@@ -164,6 +164,14 @@ typedef struct osr_state_dev {
     double pcoeff;                       /* PCoeff (CalcPCoeff, Init_Calcs.cpp:1118-1130) */
 } osr_state_dev;
 
+/* The same state with writable arrays (for osr_unpack_device). */
+typedef struct osr_state_mut_dev {
+    const int8_t *label_m, *label_f;
+    double *Po_m, *Po_f, *x_m, *y_m, *x_f, *y_f, *S_m, *S_f;
+    const double *rho_prev_m, *pre_m, *pre_f;
+    double pcoeff;
+} osr_state_mut_dev;
+
 typedef struct osr_asm osr_asm;
 
 /* Assembly context for one grid: derives the face kA/h (IFK_Calc,
@@ -200,6 +208,16 @@ int osr_jacobian_rowptr_device(osr_asm *a, const osr_state_dev *s, int64_t *rowp
 int osr_jacobian_device(osr_asm *a, const osr_state_dev *s, const int64_t *rowptr, int32_t *cols,
                         double *vals, void *stream);
 
+/* UpdateAll (PetscAUX.cpp:176-498): unknown vector X [nrows] (device) ->
+ * state, in place: mole fractions clamped to [0, 1] and renormalised, the
+ * pressure limiter (+-0.01 Pa when the new pressure exceeds the old one by
+ * more than 1 %, PetscAUX.cpp:227), the 5 MPa floor, saturations clamped and
+ * renormalised.  Layout from the last osr_asm_layout. */
+int osr_unpack_device(osr_asm *a, const osr_state_mut_dev *s, const double *X, void *stream);
+
+/* InitGuess2 (PetscAUX.cpp:878-964): state -> unknown vector X [nrows]. */
+int osr_pack_device(osr_asm *a, const osr_state_dev *s, double *X, void *stream);
+
 /* Cell properties after the sweep, [16+2nc][2n] (device): porpor, Pw, Po, Pg,
  * IFT, rho_o, rho_g, mu_o, mu_g, gamma_o, gamma_g, krw, kro, krg, Pcw, Pcg,
  * D_o[nc], D_g[nc]. */
@@ -207,7 +225,9 @@ int osr_props_device(osr_asm *a, const osr_state_dev *s, double *out, void *stre
 
 /* Launch-shape knobs (no counterpart in the reference; set by the
  * autotuner, osiris_amd/tune.py): "flash.refill_min" (1..64: idle lanes a
- * wave waits for before refilling from the work queue) and
+ * wave waits for before refilling from the work queue), "flash.park_after"
+ * (0..500: passes after which the persistent flash hands a cell to the
+ * lane-group kernel; 0 = never) and
  * "flash.blocks_per_cu" (0 = occupancy-derived; else 256-thread blocks per
  * CU of the persistent flash grid). */
 int osr_tune_set(osr_ctx *ctx, const char *key, double value);

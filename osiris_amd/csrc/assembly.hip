@@ -1256,3 +1256,140 @@ int osr_props_device(osr_asm* a, const osr_state_dev* s, double* out, void* stre
 }
 
 }  // extern "C"
+
+// ============================================================ UpdateAll
+namespace osr {
+
+constexpr double MAX_P_COEFF = 0.01;   // Functions.h:4
+constexpr double MINPPP = 5000000;     // Functions.h:259
+constexpr double EPS_COMP = 0;         // Functions.h:283
+
+// UpdateAll (PetscAUX.cpp:176-498): one lane per block reads its slice of the
+// unknown vector X (reference layout) and updates the state in place:
+// mole fractions clamped to [EPS_COMP, 1] and renormalised (uniform if they
+// sum to 0), the pressure limiter that adds +-MAX_P_COEFF Pa when the new
+// value exceeds the old by more than 1 %, the MINPPP floor, and saturations
+// clamped and renormalised.  Stale x (gas blocks) / y (liquid blocks) stay.
+template <int NC>
+__global__ void k_unpack(int n, const int64_t* boff, const int8_t* lm, const int8_t* lf, double* Po_m,
+                         double* Po_f, double* x_m, double* y_m, double* x_f, double* y_f, double* S_m,
+                         double* S_f, double pcoeff, const double* X) {
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= 2 * n) return;
+    const int k = blk / n, c = blk % n;
+    const int L = (k == 0 ? lm : lf)[c];
+    double* Po = k == 0 ? Po_m : Po_f;
+    double* x = k == 0 ? x_m : x_f;
+    double* y = k == 0 ? y_m : y_f;
+    double* S = k == 0 ? S_m : S_f;
+    const double* u = X + boff[blk];
+    int p = 0;
+    auto comp = [&](double* arr) {
+        double v[NC];
+        double sum = 0;
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            double t = u[p++];
+            if (t < EPS_COMP) t = 0;
+            else if (t > 1) t = 1;
+            v[i] = t;
+            sum += t;
+        }
+#pragma unroll
+        for (int i = 0; i < NC; i++) arr[(int64_t)i * n + c] = sum != 0 ? v[i] / sum : 1.0 / NC;
+    };
+    if (L != -1) comp(x);
+    if (L != 1) comp(y);
+    const double xp = u[p++];
+    double P = Po[c];
+    if (fabs(pcoeff * xp) > P * (1 + MAX_P_COEFF)) {
+        const double dv = xp * pcoeff - P;
+        P += MAX_P_COEFF * (dv > 0 ? 1.0 : (dv < 0 ? -1.0 : 0.0));
+    } else {
+        P = xp * pcoeff;
+    }
+    if (P < MINPPP) P = MINPPP;
+    Po[c] = P;
+    double Sw = u[p++];
+    double So = S[(int64_t)n + c], Sg = S[2 * (int64_t)n + c];
+    if (L != -1) So = u[p++];
+    if (L != 1) Sg = u[p++];
+    Sw = Sw < 0 ? 0 : (Sw > 1 ? 1 : Sw);
+    if (L != -1) So = So > 1 ? 1 : (So < 0 ? 0 : So);
+    if (L != 1) Sg = Sg > 1 ? 1 : (Sg < 0 ? 0 : Sg);
+    const double sum = Sw + (L != -1 ? So : 0.0) + (L != 1 ? Sg : 0.0);
+    if (sum != 0) {
+        Sw /= sum;
+        if (L != -1) So /= sum;
+        if (L != 1) Sg /= sum;
+    }
+    S[c] = Sw;
+    if (L != -1) S[(int64_t)n + c] = So;
+    if (L != 1) S[2 * (int64_t)n + c] = Sg;
+}
+
+// InitGuess2 (PetscAUX.cpp:878-964): state -> unknown vector X.
+template <int NC>
+__global__ void k_pack(int n, const int64_t* boff, const int8_t* lm, const int8_t* lf, const double* Po_m,
+                       const double* Po_f, const double* x_m, const double* y_m, const double* x_f,
+                       const double* y_f, const double* S_m, const double* S_f, double pcoeff, double* X) {
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= 2 * n) return;
+    const int k = blk / n, c = blk % n;
+    const int L = (k == 0 ? lm : lf)[c];
+    const double* x = k == 0 ? x_m : x_f;
+    const double* y = k == 0 ? y_m : y_f;
+    const double* S = k == 0 ? S_m : S_f;
+    double* u = X + boff[blk];
+    int p = 0;
+    if (L != -1)
+#pragma unroll
+        for (int i = 0; i < NC; i++) u[p++] = x[(int64_t)i * n + c];
+    if (L != 1)
+#pragma unroll
+        for (int i = 0; i < NC; i++) u[p++] = y[(int64_t)i * n + c];
+    u[p++] = (k == 0 ? Po_m : Po_f)[c] / pcoeff;
+    u[p++] = S[c];
+    if (L != -1) u[p++] = S[(int64_t)n + c];
+    if (L != 1) u[p++] = S[2 * (int64_t)n + c];
+}
+
+}  // namespace osr
+
+extern "C" {
+
+int osr_unpack_device(osr_asm* a, const osr_state_mut_dev* s, const double* X, void* stream) {
+    if (!a || !s || !X || !s->label_m || !s->label_f || !s->Po_m || !s->Po_f || !s->x_m || !s->y_m ||
+        !s->x_f || !s->y_f || !s->S_m || !s->S_f || !(s->pcoeff > 0))
+        return OSR_E_ARG;
+    if (a->nrows < 0) return OSR_E_ARG;
+    const int nb = 2 * a->n;
+    hipStream_t st = (hipStream_t)stream;
+#define UNPACK(NC) case NC: hipLaunchKernelGGL(osr::k_unpack<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, a->n, a->boff, \
+        s->label_m, s->label_f, s->Po_m, s->Po_f, s->x_m, s->y_m, s->x_f, s->y_f, s->S_m, s->S_f, s->pcoeff, X); break;
+    switch (a->nc) {
+        UNPACK(2) UNPACK(3) UNPACK(4) UNPACK(5) UNPACK(6) UNPACK(8) UNPACK(10) UNPACK(12)
+        default: return OSR_E_NC;
+    }
+#undef UNPACK
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+int osr_pack_device(osr_asm* a, const osr_state_dev* s, double* X, void* stream) {
+    if (!a || !s || !X || !s->label_m || !s->label_f || !s->Po_m || !s->Po_f || !s->x_m || !s->y_m ||
+        !s->x_f || !s->y_f || !s->S_m || !s->S_f || !(s->pcoeff > 0))
+        return OSR_E_ARG;
+    if (a->nrows < 0) return OSR_E_ARG;
+    const int nb = 2 * a->n;
+    hipStream_t st = (hipStream_t)stream;
+#define PACK(NC) case NC: hipLaunchKernelGGL(osr::k_pack<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, a->n, a->boff, \
+        s->label_m, s->label_f, s->Po_m, s->Po_f, s->x_m, s->y_m, s->x_f, s->y_f, s->S_m, s->S_f, s->pcoeff, X); break;
+    switch (a->nc) {
+        PACK(2) PACK(3) PACK(4) PACK(5) PACK(6) PACK(8) PACK(10) PACK(12)
+        default: return OSR_E_NC;
+    }
+#undef PACK
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+}  // extern "C"

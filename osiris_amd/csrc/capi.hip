@@ -70,7 +70,7 @@ int osr_ctx_create(const osr_fluid* fluid, osr_ctx** out) {
     }
     c->num_cu = prop.multiProcessorCount;
     if (hipMalloc(&c->d_fluid, sizeof(osr::FluidDev)) != hipSuccess ||
-        hipMalloc(&c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess ||
         hipMemcpy(c->d_fluid, &c->host_fluid, sizeof(osr::FluidDev), hipMemcpyHostToDevice) != hipSuccess) {
         osr_ctx_destroy(c);
         return OSR_E_HIP;
@@ -87,6 +87,7 @@ int osr_ctx_destroy(osr_ctx* ctx) {
     if (ctx->d_fluid) hipFree(ctx->d_fluid);
     if (ctx->d_counters) hipFree(ctx->d_counters);
     if (ctx->d_queue) hipFree(ctx->d_queue);
+    if (ctx->d_park) hipFree(ctx->d_park);
     hipSetDevice(dev);
     delete ctx;
     return OSR_OK;
@@ -110,6 +111,18 @@ int osr_tune_set(osr_ctx* ctx, const char* key, double value) {
     if (k == "flash.refill_min") {
         if (value < 1 || value > 64) return OSR_E_ARG;
         ctx->refill_min = (int)value;
+    } else if (k == "flash.mixed_warm_iters") {
+        if (value < 1 || value > 100) return OSR_E_ARG;
+        ctx->mixed_warm_iters = (int)value;
+    } else if (k == "flash.mixed_tol32") {
+        if (!(value > 0 && value < 1)) return OSR_E_ARG;
+        ctx->mixed_tol32 = (float)value;
+    } else if (k == "flash.mixed_max32") {
+        if (value < 1 || value > 500) return OSR_E_ARG;
+        ctx->mixed_max32 = (int)value;
+    } else if (k == "flash.park_after") {
+        if (value < 0 || value > 500) return OSR_E_ARG;
+        ctx->park_after = (int)value;
     } else if (k == "flash.blocks_per_cu") {
         if (value < 0 || value > 64) return OSR_E_ARG;
         ctx->flash_blocks_per_cu = (int)value;

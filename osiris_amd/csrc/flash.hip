@@ -30,9 +30,26 @@ struct FlashArgs {
     const double* z;
     double sw;
     osr_flash_out out;
-    unsigned long long* counter;
+    unsigned long long* counter;   // [0] work-queue head, [1] park-queue tail, [2] park-queue head
     int refill_min;
+    int park_after;                // 0: never park; else park cells still iterating after this many passes
+    double* park;                  // parked-cell store, [NC + 5][cap] (K, L, Lb, it, rr, cell)
+    int64_t park_cap;
+    // mixed precision (park slots q index every array below)
+    double* kb;                    // [NC][cap] K after the FP32 stage
+    double* itb;                   // [2][cap] passes and RR iterations after the FP32 stage
+    int64_t* list2;                // slots for the FP64 Newton polish
+    int64_t* list3;                // slots sent back to FP64 SS from the park state
+    const int64_t* glist;          // group kernel: slots to finish (nullptr = every parked slot)
+    int glen_slot;                 // counter holding the length of glist (or of the park queue)
+    float tol32;                   // FP32 stage stop: max_i |K_i'/K_i - 1| below this
+    int max32;                     // FP32 stage pass cap
 };
+
+// counter slots
+constexpr int CNT_WORK = 0, CNT_PARK = 1, CNT_GROUP = 2, CNT_B = 3, CNT_NT_TAIL = 4, CNT_NT_HEAD = 5,
+              CNT_FB_TAIL = 6;
+constexpr int NCOUNTERS = 16;
 
 template <int NC, class R>
 struct SSCell {
@@ -169,7 +186,7 @@ __device__ __forceinline__ void cell_write(const FlashArgs& a, int64_t c, const 
 
 // ------------------------------------------------------------ plain form
 template <int NC>
-__global__ __launch_bounds__(256) void k_flash_plain(const FluidDev* __restrict__ F, FlashArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_flash_plain(const FluidDev* __restrict__ F, FlashArgs a) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n) return;
     const FluidConsts<double>& C = F->d;
@@ -190,7 +207,7 @@ __device__ __forceinline__ unsigned lane_id() {
 }
 
 template <int NC>
-__global__ __launch_bounds__(256) void k_flash_persistent(const FluidDev* __restrict__ F, FlashArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_flash_persistent(const FluidDev* __restrict__ F, FlashArgs a) {
     const FluidConsts<double>& C = F->d;
     const bool srk = F->eos == OSR_EOS_SRK;
     const unsigned lane = lane_id();
@@ -207,7 +224,7 @@ __global__ __launch_bounds__(256) void k_flash_persistent(const FluidDev* __rest
         if (!exhausted && nidle >= refill_min) {
             const int leader = __ffsll((long long)idle) - 1;
             unsigned long long base = 0;
-            if ((int)lane == leader) base = atomicAdd(a.counter, (unsigned long long)nidle);
+            if ((int)lane == leader) base = atomicAdd(a.counter + CNT_WORK, (unsigned long long)nidle);
             base = __shfl(base, leader, 64);
             if (base + (unsigned long long)nidle >= (unsigned long long)a.n) exhausted = true;
             if (!active) {
@@ -220,19 +237,647 @@ __global__ __launch_bounds__(256) void k_flash_persistent(const FluidDev* __rest
             }
         }
         if (__ballot(active) == 0ull) break;
+        bool park = false;
         if (active) {
             const double esum = ss_pass<NC, double>(C, srk, s, x, y);
             if (!(esum > FLASH_TOL && s.it < MAX_SS_ITER)) {
                 cell_write<NC>(a, c, s, x, y);
+                active = false;
+            } else if (a.park_after > 0 && s.it >= a.park_after) {
+                park = true;
+            }
+        }
+        // Slow cells move to the park queue (one atomic per wave) and are
+        // finished by the lane-group kernel, which runs a pass with far
+        // shorter latency than one lane can.
+        const unsigned long long pk = __ballot(park);
+        if (pk) {
+            const int leader = __ffsll((long long)pk) - 1;
+            unsigned long long base = 0;
+            if ((int)lane == leader) base = atomicAdd(a.counter + CNT_PARK, (unsigned long long)__popcll(pk));
+            base = __shfl(base, leader, 64);
+            if (park) {
+                const int64_t q = (int64_t)base + __popcll(pk & lt_mask);
+                const int64_t cap = a.park_cap;
+#pragma unroll
+                for (int i = 0; i < NC; i++) a.park[(int64_t)i * cap + q] = s.K[i];
+                a.park[(int64_t)(NC + 0) * cap + q] = s.L;
+                a.park[(int64_t)(NC + 1) * cap + q] = s.Lb;
+                a.park[(int64_t)(NC + 2) * cap + q] = (double)s.it;
+                a.park[(int64_t)(NC + 3) * cap + q] = (double)s.rr;
+                a.park[(int64_t)(NC + 4) * cap + q] = (double)c;
                 active = false;
             }
         }
     }
 }
 
+// ------------------------------------------------------- lane-group form
+// G = 2S lanes finish one parked cell: lanes [0, S) evaluate the liquid
+// phase, [S, 2S) the vapour phase, slot j of each half owns component j.
+// Rachford-Rice sums and mixing sums are xor-butterflies inside a half (each
+// lane adds the same operands in the same pairing, so every lane of the group
+// holds bitwise the same L and the group's control flow is uniform); the
+// mixing matrix-vector product gathers the half's S compositions by shuffle.
+// A pass then costs one cubic, two logs and one exp per lane instead of a
+// whole cell's worth per lane, which is what bounds the slowest cells.
+// Contraction stays off for every operand of a butterfly: a product fused
+// into the first add (fma(z, td, partner's rounded z*td)) rounds differently
+// on the two lanes of a pair and the lanes of a group would then disagree.
+template <int S>
+__device__ __forceinline__ double half_sum(double v) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One atomic per wave whose result is wave-uniform by construction (SGPR),
+// so loops that exit on it have a uniform exit and the compiler keeps every
+// lane of the wave on the same iteration (cross-lane reads inside the loop
+// then always see their partners' current values).  All 64 lanes must be
+// active at the call.
+__device__ __forceinline__ unsigned long long wave_fetch_add(unsigned long long* p, unsigned long long v) {
+    unsigned long long r = 0;
+    if (lane_id() == 0) r = atomicAdd(p, v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(r & 0xffffffffull));
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(r >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+template <int NC, int S>
+__global__ __launch_bounds__(256) void k_flash_group(const FluidDev* __restrict__ F, FlashArgs a) {
+#pragma clang fp contract(off)
+    constexpr int G = 2 * S;
+    const FluidConsts<double>& C = F->d;
+    const bool srk = F->eos == OSR_EOS_SRK;
+    const int lane = (int)lane_id();
+    const int gl = lane % G;
+    const int gbase = lane - gl;
+    const int h = gl / S;          // 0 liquid, 1 vapour
+    const int j = gl % S;          // component slot
+    const bool valid = j < NC;
+    const int jc = valid ? j : 0;
+    constexpr int GPW = 64 / G;    // groups per wave
+    const unsigned long long qlen = a.counter[a.glen_slot];
+    const int64_t cap = a.park_cap;
+    const double bj = C.b[jc];
+    while (true) {
+        const unsigned long long base = wave_fetch_add(a.counter + CNT_GROUP, (unsigned long long)GPW);
+        if (base >= qlen) break;   // wave-uniform
+        const unsigned long long qi = base + (unsigned long long)(lane / G);
+        if (qi >= qlen) continue;  // group-uniform: this group has no cell this round
+        const int64_t q = a.glist ? a.glist[qi] : (int64_t)qi;
+        const int64_t c = (int64_t)a.park[(int64_t)(NC + 4) * cap + q];
+        const double T = a.T ? a.T[c] : a.T0;
+        const double P = a.P[c];
+        const double z = valid ? a.z[(int64_t)jc * a.n + c] : 0.0;
+        double K = valid ? a.park[(int64_t)jc * cap + q] : 0.0;
+        double L = a.park[(int64_t)(NC + 0) * cap + q];
+        double Lb = a.park[(int64_t)(NC + 1) * cap + q];
+        int it = (int)a.park[(int64_t)(NC + 2) * cap + q];
+        int rr = (int)a.park[(int64_t)(NC + 3) * cap + q];
+        const double sa = valid ? C.sac[jc] * (1.0 + C.m[jc] * (1.0 - sqrt(T / C.Tc[jc]))) : 0.0;
+        const double rt = RGAS * T;
+        const double pB = P / rt, pA = pB / rt;
+        double x = 0, y = 0, Zl = 0, Zg = 0;
+        double esum;
+        do {
+            // Rachford-Rice Newton from 0.6
+            double Lr = 0.6, step;
+            int jr = 0;
+            do {
+                double td = 0;
+                if (valid) td = (K >= 0) ? (1.0 - K) / (Lr + (1.0 - Lr) * K) : 1.0 / (Lr - 1.0);
+                const double Fs = half_sum<S>(z * td);
+                const double dF = -half_sum<S>(z * td * td);
+                step = Fs / dF;
+                Lr -= step;
+                jr++;
+            } while (step * step > NEWTON_TOL && jr < MAX_FLASH_N_ITR);
+            rr += jr;
+            if (Lr > 1) Lr = 1;
+            else if (Lr < 0) Lr = 0;
+            esum = fabs(Lb - Lr);
+            Lb = Lr;
+            L = Lr;
+            if (K >= 0) {
+                x = z / (L + K * (1.0 - L));
+                y = K * x;
+            } else {
+                x = 0;
+                y = z / (1.0 - L);
+            }
+            if (!valid) x = y = 0;
+            const double cph = h == 0 ? x : y;
+            const double u = sa * cph;
+            double sacc = 0;
+#pragma unroll
+            for (int k2 = 0; k2 < NC; k2++) sacc += C.bic[jc * NC + k2] * __shfl(u, gbase + h * S + k2, 64);
+            const double t = valid ? sa * sacc : 0.0;
+            const double Ca = half_sum<S>(u * sacc);
+            const double Cb = half_sum<S>(cph * (valid ? bj : 0.0));
+            PhaseInv<double> ph;
+            ph.A = Ca * pA;
+            ph.B = Cb * pB;
+            ph.Z = phase_z<double>(srk, ph.A, ph.B, h == 1);
+            ph.invCa = 1.0 / Ca;
+            ph.invCb = 1.0 / Cb;
+            ph.lnZB = log(ph.Z - ph.B);
+            const double DD = ph.A / (ph.B * (C.d1 - C.d2));
+            ph.DDFF = DD * log((ph.Z + C.d2 * ph.B) / (ph.Z + C.d1 * ph.B));
+            const double lp = lnphi<double>(ph, bj, t);
+            const double lo = __shfl_xor(lp, S, 64);
+            const double Zo = __shfl_xor(ph.Z, S, 64);
+            Zl = h == 0 ? ph.Z : Zo;
+            Zg = h == 0 ? Zo : ph.Z;
+            const double av = h == 0 ? lp : lo;   // ln phi liquid of component j
+            const double gv = h == 0 ? lo : lp;   // ln phi vapour of component j
+            if (valid && z != 0) {
+                double r;
+                if (K > 0 && x != 0 && isfinite(x)) {
+                    r = exp(av - gv);
+                } else {
+                    const double fL = x * exp(av);
+                    const double fG = y * exp(gv);
+                    r = (fG != 0) ? K * (fL / fG) : -1.0;
+                }
+                if (av != av) r = -1;
+                if (gv != gv) r = 1;
+                K = r;
+            }
+            it++;
+        } while (esum > FLASH_TOL && it < MAX_SS_ITER);
+        // outputs (Init_Calcs.cpp:717-736)
+        const osr_flash_out& o = a.out;
+        const int64_t n = a.n;
+        if (h == 0 && valid) {
+            if (o.x) o.x[(int64_t)jc * n + c] = x;
+            if (o.y) o.y[(int64_t)jc * n + c] = y;
+            if (o.K) o.K[(int64_t)jc * n + c] = K;
+        }
+        if (gl == 0) {
+            int8_t lab = 0;
+            if (fabs(L) < DZERO) lab = -1;
+            else if (fabs(1 - L) < DZERO) lab = 1;
+            if (o.L) o.L[c] = L;
+            if (o.label) o.label[c] = lab;
+            if (o.Zl) o.Zl[c] = Zl;
+            if (o.Zg) o.Zg[c] = Zg;
+            const double newD = Zl * L / (Zl * L + Zg * (1 - L));
+            if (o.so) o.so[c] = newD * (1 - a.sw);
+            if (o.sg) o.sg[c] = (1 - newD) * (1 - a.sw);
+            if (o.rho_l) o.rho_l[c] = P / (Zl * RGAS * T);
+            if (o.rho_g) o.rho_g[c] = P / (Zg * RGAS * T);
+            if (o.iters) o.iters[c] = it;
+            if (o.rr_iters) o.rr_iters[c] = rr;
+        }
+    }
+}
+
+
+// ------------------------------------------------------- mixed precision
+// Cells still iterating after the warm FP64 passes (parked above) run:
+//  stage B  FP32 successive substitution (one lane per cell, work queue over
+//           the park slots) until max_i |K_i'/K_i - 1| < tol32.  Cells whose
+//           L comes within 1e-3 of 0 or 1, that stop improving, or that turn
+//           non-finite go back to FP64 SS from their park state (the
+//           reference trajectory), finished by the lane-group kernel;
+//  stage C  FP64 Newton on ln K for the fixed point of the reference's own
+//           SS map: L(K) is the Rachford-Rice Newton from 0.6 with the
+//           reference's stopping rule, differentiated forward through its
+//           iterations (that RR stops at |step| <= 1e-4, so the reference
+//           converges to the fixed point of this inexact map, not to exact
+//           fugacity equality -- a polish towards the latter would differ by
+//           up to ~1e-7).  Residual and update are FP64; the Jacobian is
+//           assembled analytically (derivatives of the cubic-EOS ln phi in x)
+//           and factorised in FP32 in LDS, which only affects the rate.  One
+//           final FP64 reference pass then produces the outputs with
+//           SuccFalsh's semantics (x, y from K, K updated, L, Z, labels).
+__device__ __forceinline__ int64_t wave_push(unsigned long long* tail, bool want, unsigned lane,
+                                             unsigned long long lt_mask) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return -1;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if ((int)lane == leader) base = atomicAdd(tail, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return want ? (int64_t)base + __popcll(m & lt_mask) : -1;
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_mixed_ss32(const FluidDev* __restrict__ F, FlashArgs a) {
+    const FluidConsts<float>& C = F->f;
+    const bool srk = F->eos == OSR_EOS_SRK;
+    const unsigned lane = lane_id();
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t len = (int64_t)a.counter[CNT_PARK];
+    const int64_t cap = a.park_cap;
+    SSCell<NC, float> s;
+    float x[NC], y[NC];
+    int64_t q = -1;
+    int npass = 0;
+    bool active = false;
+    bool exhausted = len == 0;
+    while (true) {
+        const unsigned long long idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && nidle >= 16) {
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0;
+            if ((int)lane == leader) base = atomicAdd(a.counter + CNT_B, (unsigned long long)nidle);
+            base = __shfl(base, leader, 64);
+            if (base + (unsigned long long)nidle >= (unsigned long long)len) exhausted = true;
+            if (!active) {
+                const int64_t mine = (int64_t)base + __popcll(idle & lt_mask);
+                if (mine < len) {
+                    q = mine;
+                    const int64_t c = (int64_t)a.park[(int64_t)(NC + 4) * cap + q];
+                    const double T = a.T ? a.T[c] : a.T0;
+                    const double P = a.P[c];
+                    s.T = (float)T;
+                    s.P = (float)P;
+#pragma unroll
+                    for (int i = 0; i < NC; i++) {
+                        s.z[i] = (float)a.z[(int64_t)i * a.n + c];
+                        s.K[i] = (float)a.park[(int64_t)i * cap + q];
+                    }
+                    sqrt_a<NC, float>(C, s.T, s.sa);
+                    const double rt = RGAS * T;
+                    s.pB = (float)(P / rt);
+                    s.pA = (float)(P / rt / rt);
+                    s.L = (float)a.park[(int64_t)(NC + 0) * cap + q];
+                    s.Lb = (float)a.park[(int64_t)(NC + 1) * cap + q];
+                    s.it = (int)a.park[(int64_t)(NC + 2) * cap + q];
+                    s.rr = (int)a.park[(int64_t)(NC + 3) * cap + q];
+                    s.Zl = s.Zg = 0;
+                    npass = 0;
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        bool to_newton = false, to_fallback = false;
+        if (active) {
+            float Kold[NC];
+#pragma unroll
+            for (int i = 0; i < NC; i++) Kold[i] = s.K[i];
+            ss_pass<NC, float>(C, srk, s, x, y);
+            npass++;
+            float dk = 0.f;
+            bool ok = true;
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+                if (s.z[i] != 0.f) {
+                    ok = ok && (s.K[i] > 0.f) && isfinite(s.K[i]);
+                    dk = fmaxf(dk, fabsf(s.K[i] / Kold[i] - 1.f));
+                }
+            }
+            const float L = s.L;
+            if (!ok || !(L > 1e-3f && L < 1.f - 1e-3f)) to_fallback = true;
+            else if (dk < a.tol32) to_newton = true;
+            else if (npass >= a.max32) to_fallback = true;
+        }
+        const int64_t p2 = wave_push(a.counter + CNT_NT_TAIL, to_newton, lane, lt_mask);
+        if (to_newton) {
+            a.list2[p2] = q;
+#pragma unroll
+            for (int i = 0; i < NC; i++) a.kb[(int64_t)i * cap + q] = (double)s.K[i];
+            a.itb[q] = (double)s.it;
+            a.itb[cap + q] = (double)s.rr;
+            active = false;
+        }
+        const int64_t p3 = wave_push(a.counter + CNT_FB_TAIL, to_fallback, lane, lt_mask);
+        if (to_fallback) {
+            a.list3[p3] = q;
+            active = false;
+        }
+    }
+}
+
+// Per-phase pieces of d ln phi_i / d x_k for the cubic EOS as written in the
+// reference (ln phi_i = (b_i/Cb)(Z-1) - ln(Z-B) + q Lam (2 t_i/Ca - b_i/Cb),
+// q = A/(B(d1-d2)), Lam = ln((Z+d2 B)/(Z+d1 B)), A = Ca pA, B = Cb pB):
+//   Phi_ik = (b_i/Cb) U_k - V_k + E_i W_k - c1 t_i t_k + c2 a_ik
+// with E_i = 2 t_i/Ca - b_i/Cb, a_ik = sa_i sa_k bic_ik, c1 = 4 q Lam/Ca^2,
+// c2 = 2 q Lam/Ca, and per column k (Z_k from the implicit cubic):
+//   U_k = Z_k - b_k (Z-1)/Cb + q Lam b_k/Cb,  V_k = (Z_k - b_k pB)/(Z - B),
+//   W_k = Lam q_k + q Lam_k.
+template <int NC>
+struct PhaseDeriv {
+    float U[NC], V[NC], W[NC], t[NC];
+    float c1, c2, iCb, iCa;
+};
+
+template <int NC>
+__device__ __forceinline__ void phase_lnphi_deriv(const FluidConsts<double>& C, bool srk, const double (&sa)[NC],
+                                                  const double (&x)[NC], double pA, double pB, bool gas,
+                                                  double (&lnp)[NC], PhaseDeriv<NC>& D) {
+    double t[NC];
+    const PhaseInv<double> p = phase_eval<NC, double>(C, srk, sa, x, pA, pB, gas, t);
+#pragma unroll
+    for (int i = 0; i < NC; i++) lnp[i] = lnphi<double>(p, C.b[i], t[i]);
+    const double Z = p.Z, A = p.A, B = p.B;
+    const double u = srk ? 1.0 : 2.0, w = srk ? 0.0 : 1.0;
+    const double c2 = -1.0 + (u - 1.0) * B;
+    const double c1 = A - u * B * B - u * B - w * B * B;
+    const double FZ = 3.0 * Z * Z + 2.0 * c2 * Z + c1;
+    const double FA = Z - B;
+    const double FB = (u - 1.0) * Z * Z - (2.0 * (u + w) * B + u) * Z - A + 3.0 * w * B * B + 2.0 * w * B;
+    const double q = A / (B * (C.d1 - C.d2));
+    const double zd1 = Z + C.d1 * B, zd2 = Z + C.d2 * B;
+    const double Lam = log(zd2 / zd1);
+    const double iCa = p.invCa, iCb = p.invCb;
+    const double iFZ = 1.0 / FZ, iZB = 1.0 / (Z - B), iA = 1.0 / A, iB = 1.0 / B;
+    const double qL = q * Lam;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const double Ak = 2.0 * t[k] * pA;
+        const double Bk = C.b[k] * pB;
+        const double Zk = -(FA * Ak + FB * Bk) * iFZ;
+        const double qk = q * (Ak * iA - Bk * iB);
+        const double Lk = (Zk + C.d2 * Bk) / zd2 - (Zk + C.d1 * Bk) / zd1;
+        D.U[k] = (float)(Zk - C.b[k] * (Z - 1.0) * iCb + qL * C.b[k] * iCb);
+        D.V[k] = (float)((Zk - Bk) * iZB);
+        D.W[k] = (float)(Lam * qk + q * Lk);
+        D.t[k] = (float)t[k];
+    }
+    D.c1 = (float)(4.0 * qL * iCa * iCa);
+    D.c2 = (float)(2.0 * qL * iCa);
+    D.iCb = (float)iCb;
+    D.iCa = (float)iCa;
+}
+
+// LDS row layout of the Newton system, per lane: columns [0, NC) J, NC the
+// right-hand side -g, NC+1 the rank-1 coefficient r_i, NC+2 the phase's t_i.
+template <int NC> struct NewtonLds { static constexpr int W = NC + 3; };
+
+// J[i][k] += sgn * Phi_ik * dd_k and r_i += sgn * sum_k Phi_ik uu_k.  The
+// column factors are folded into register vectors first (U_k dd_k, ...); the
+// row loop is not unrolled and reads its per-row values from LDS and from
+// scalar constant loads, so only O(NC) values are live at a time.
+template <int NC>
+__device__ __forceinline__ void add_phase_block(const FluidConsts<float>& Cf, const PhaseDeriv<NC>& D,
+                                                float Tf, const float (&dd)[NC], const float (&uu)[NC],
+                                                float sgn, float* sJ, int lane) {
+    constexpr int LW = NewtonLds<NC>::W;
+    float Uk[NC], Vk[NC], Wk[NC], Tk[NC], Sk[NC], Su[NC];
+    float Uu = 0, Vu = 0, Wu = 0, Tu = 0;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        Uu += D.U[k] * uu[k];
+        Vu += D.V[k] * uu[k];
+        Wu += D.W[k] * uu[k];
+        Tu += D.t[k] * uu[k];
+        const float f = sgn * dd[k];
+        Uk[k] = D.U[k] * f;
+        Vk[k] = D.V[k] * f;
+        Wk[k] = D.W[k] * f;
+        Tk[k] = D.t[k] * f;
+        const float sak = Cf.sac[k] * (1.f + Cf.m[k] * (1.f - sqrtf(Tf / Cf.Tc[k])));
+        Sk[k] = sak * f;
+        Su[k] = sak * uu[k];
+        sJ[(k * LW + NC + 2) * 64 + lane] = D.t[k];
+    }
+#pragma unroll 1
+    for (int i = 0; i < NC; i++) {
+        const float bi = Cf.b[i] * D.iCb;
+        const float ti = sJ[(i * LW + NC + 2) * 64 + lane];
+        const float Ei = 2.f * ti * D.iCa - bi;
+        const float sai = Cf.sac[i] * (1.f + Cf.m[i] * (1.f - sqrtf(Tf / Cf.Tc[i])));
+        const float c2i = D.c2 * sai;
+        const float c1i = D.c1 * ti;
+        float bicu = 0;
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const float bik = Cf.bic[i * NC + k];
+            sJ[(i * LW + k) * 64 + lane] += bi * Uk[k] - Vk[k] + Ei * Wk[k] - c1i * Tk[k] + c2i * bik * Sk[k];
+            bicu += bik * Su[k];
+        }
+        sJ[(i * LW + NC + 1) * 64 + lane] += sgn * (bi * Uu - Vu + Ei * Wu - c1i * Tu + c2i * bicu);
+    }
+}
+
+constexpr int MIXED_NEWTON_MAX = 8;
+
+// Stage C.  One lane per cell, per-lane work loop (no cross-lane operation);
+// J [NC][NC+1] (augmented with -g) lives in LDS as FP32, column-interleaved
+// by lane so the 64 lanes of the (single-wave) block never share a bank row.
+template <int NC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
+    constexpr int LW = NewtonLds<NC>::W;
+    __shared__ float sJ[NC * LW * 64];
+    const FluidConsts<double>& C = F->d;
+    const FluidConsts<float>& Cf = F->f;
+    const bool srk = F->eos == OSR_EOS_SRK;
+    const int lane = threadIdx.x;
+    const int64_t len = (int64_t)a.counter[CNT_NT_TAIL];
+    const int64_t cap = a.park_cap;
+#define JM(r, c) sJ[((r) * LW + (c)) * 64 + lane]
+    // per-lane work loop with no cross-lane operation (each lane's atomic is
+    // its own; the compiler merges them into one per wave)
+    while (true) {
+        const int64_t item = (int64_t)atomicAdd(a.counter + CNT_NT_HEAD, 1ull);
+        if (item >= len) break;
+        const int64_t q = a.list2[item];
+        const int64_t c = (int64_t)a.park[(int64_t)(NC + 4) * cap + q];
+        const double T = a.T ? a.T[c] : a.T0;
+        const double P = a.P[c];
+        double z[NC], K[NC], sa[NC];
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            z[i] = a.z[(int64_t)i * a.n + c];
+            K[i] = a.kb[(int64_t)i * cap + q];
+        }
+        sqrt_a<NC, double>(C, T, sa);
+        const double rt = RGAS * T;
+        const double pB = P / rt, pA = pB / rt;
+        int rr = (int)a.itb[cap + q];
+        bool ok = false;
+        int itn = 0;
+        for (; itn < MIXED_NEWTON_MAX; itn++) {
+            // Rachford-Rice exactly as the reference pass, with dL/dlnK carried along
+            double L = 0.6, step;
+            double dL[NC];
+#pragma unroll
+            for (int m = 0; m < NC; m++) dL[m] = 0;
+            int j = 0;
+            do {
+                double Fs = 0, dF = 0, sB = 0;
+#pragma unroll
+                for (int i = 0; i < NC; i++) {
+                    const double td = (1.0 - K[i]) / (L + (1.0 - L) * K[i]);
+                    Fs += z[i] * td;
+                    dF -= z[i] * td * td;
+                    sB -= z[i] * td * td * td;
+                }
+                step = Fs / dF;
+                const double idF2 = 1.0 / (dF * dF);
+#pragma unroll
+                for (int m = 0; m < NC; m++) {
+                    // d td_i/dL = -td_i^2, d td_m/d lnK_m = -K_m/D_m^2
+                    const double D = L + (1.0 - L) * K[m];
+                    const double td = (1.0 - K[m]) / D;
+                    const double gK = -z[m] * K[m] / (D * D);
+                    const double dFm = dF * dL[m] + gK;
+                    const double ddFm = -2.0 * (sB * dL[m] + td * gK);
+                    dL[m] -= (dFm * dF - Fs * ddFm) * idF2;
+                }
+                L -= step;
+                j++;
+            } while (step * step > NEWTON_TOL && j < MAX_FLASH_N_ITR);
+            rr += j;
+            if (!(L > 1e-3 && L < 1.0 - 1e-3)) break;   // near or past a boundary: FP64 SS decides
+            double x[NC], g[NC];
+            float dd[NC], uu[NC], w[NC];
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+                const double D = L + K[i] * (1.0 - L);
+                x[i] = z[i] / D;
+                const double iD2 = 1.0 / (D * D);
+                dd[i] = (float)(-z[i] * K[i] * (1.0 - L) * iD2);   // d x_i / d lnK_i (L fixed)
+                uu[i] = (float)(-z[i] * (1.0 - K[i]) * iD2);       // d x_i / d L
+                w[i] = (float)dL[i];
+            }
+#pragma unroll
+            for (int i = 0; i < NC; i++)
+#pragma unroll
+                for (int k = 0; k <= NC + 1; k++) JM(i, k) = (i == k) ? 1.f : 0.f;
+            const float Tf = (float)T;
+            {
+                PhaseDeriv<NC> D;
+                phase_lnphi_deriv<NC>(C, srk, sa, x, pA, pB, false, g, D);
+                add_phase_block<NC>(Cf, D, Tf, dd, uu, -1.f, sJ, lane);
+            }
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+                g[i] = log(K[i]) - g[i];                         // ln K_i - ln phiL_i
+                dd[i] = (float)K[i] * (dd[i] + (float)x[i]);     // d y_i / d lnK_i
+                uu[i] = (float)K[i] * uu[i];                     // d y_i / d L
+                x[i] = K[i] * x[i];                              // y
+            }
+            {
+                double lg[NC];
+                PhaseDeriv<NC> D;
+                phase_lnphi_deriv<NC>(C, srk, sa, x, pA, pB, true, lg, D);
+                add_phase_block<NC>(Cf, D, Tf, dd, uu, 1.f, sJ, lane);
+#pragma unroll
+                for (int i = 0; i < NC; i++) g[i] += lg[i];
+            }
+            bool fin = true;
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+                const double gi = (z[i] != 0) ? g[i] : 0.0;
+                fin = fin && isfinite(gi);
+                const float ri = JM(i, NC + 1);
+#pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if (z[i] == 0) JM(i, k) = (i == k) ? 1.f : 0.f;
+                    else JM(i, k) += ri * w[k];
+                }
+                JM(i, NC) = (float)(-gi);
+            }
+            if (!fin) break;
+            // LU with partial pivoting on the augmented [J | -g]
+            bool sing = false;
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                int piv = k;
+                float best = fabsf(JM(k, k));
+#pragma unroll
+                for (int rw = k + 1; rw < NC; rw++) {
+                    const float v = fabsf(JM(rw, k));
+                    if (v > best) { best = v; piv = rw; }
+                }
+                if (!(best > 0.f)) sing = true;
+                if (piv != k) {
+#pragma unroll
+                    for (int cc = k; cc <= NC; cc++) {
+                        const float t0 = JM(k, cc);
+                        JM(k, cc) = JM(piv, cc);
+                        JM(piv, cc) = t0;
+                    }
+                }
+                const float ip = 1.f / JM(k, k);
+#pragma unroll
+                for (int rw = k + 1; rw < NC; rw++) {
+                    const float f = JM(rw, k) * ip;
+#pragma unroll
+                    for (int cc = k + 1; cc <= NC; cc++) JM(rw, cc) -= f * JM(k, cc);
+                }
+            }
+            if (sing) break;
+            float dlt[NC];
+#pragma unroll
+            for (int k = NC - 1; k >= 0; k--) {
+                float acc = JM(k, NC);
+#pragma unroll
+                for (int cc = k + 1; cc < NC; cc++) acc -= JM(k, cc) * dlt[cc];
+                dlt[k] = acc / JM(k, k);
+            }
+            double dmax = 0;
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+                if (z[i] != 0) {
+                    K[i] *= exp((double)dlt[i]);
+                    dmax = fmax(dmax, fabs((double)dlt[i]));
+                }
+            }
+            if (!isfinite(dmax) || dmax > 1.0) break;
+            if (dmax < 1e-9) {
+                ok = true;
+                itn++;
+                break;
+            }
+        }
+#undef JM
+        if (!ok) {
+            const unsigned long long p3 = atomicAdd(a.counter + CNT_FB_TAIL, 1ull);
+            a.list3[p3] = q;
+            continue;
+        }
+        // final reference pass from the polished K
+        SSCell<NC, double> s;
+        double x[NC], y[NC];
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            s.K[i] = K[i];
+            s.z[i] = z[i];
+            s.sa[i] = sa[i];
+        }
+        s.T = T;
+        s.P = P;
+        s.pA = pA;
+        s.pB = pB;
+        s.L = 0;
+        s.Lb = 10;
+        s.Zl = s.Zg = 0;
+        s.it = (int)a.itb[q] + itn;
+        s.rr = rr;
+        ss_pass<NC, double>(C, srk, s, x, y);
+        cell_write<NC>(a, c, s, x, y);
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int count) {
     const int i = threadIdx.x;
     if (i < count) p[i] = 0ull;
+}
+
+// Workspace for parked cells: park store [NC + 5][n], then (mixed) kb
+// [NC][n], itb [2][n], list2 [n], list3 [n] -- every cell parks at most once.
+static int ensure_park(osr_ctx* ctx, int nc, int64_t n, bool mixed) {
+    const size_t words = (size_t)(nc + 5) * (size_t)n + (mixed ? (size_t)(nc + 4) * (size_t)n : 0);
+    const size_t need = words * sizeof(double);
+    if (need > ctx->park_bytes) {
+        if (ctx->d_park) (void)hipFree(ctx->d_park);
+        ctx->d_park = nullptr;
+        ctx->park_bytes = 0;
+        OSR_HIP(hipMalloc(&ctx->d_park, need));
+        ctx->park_bytes = need;
+    }
+    return OSR_OK;
 }
 
 template <int NC>
@@ -252,9 +897,54 @@ static int launch_nc(osr_ctx* ctx, const FlashArgs& a, int mode, hipStream_t str
     const int64_t need = (a.n + block - 1) / block;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, ctx->d_counters, 8);
+    const bool mixed = mode == OSR_FLASH_MIXED;
+    FlashArgs b = a;
+    const int park_after = mixed ? (ctx->mixed_warm_iters < 1 ? 1 : ctx->mixed_warm_iters) : ctx->park_after;
+    if (park_after > 0) {
+        OSR_HIP_RET(ensure_park(ctx, NC, a.n, mixed));
+        double* base = (double*)ctx->d_park;
+        const int64_t n = a.n;
+        b.park_after = park_after;
+        b.park = base;
+        b.park_cap = n;
+        if (mixed) {
+            b.kb = base + (int64_t)(NC + 5) * n;
+            b.itb = b.kb + (int64_t)NC * n;
+            b.list2 = (int64_t*)(b.itb + 2 * n);
+            b.list3 = b.list2 + n;
+            b.glist = b.list3;
+            b.glen_slot = CNT_FB_TAIL;
+        }
+    }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, ctx->d_counters, NCOUNTERS);
     hipLaunchKernelGGL(k_flash_persistent<NC>, dim3((unsigned)grid), dim3(block), 0, stream,
-                       ctx->d_fluid, a);
+                       ctx->d_fluid, b);
+    if (mixed) {
+        int pb = 0;
+        OSR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pb, k_mixed_ss32<NC>, block, 0));
+        if (pb < 1) pb = 1;
+        int64_t gB = (int64_t)pb * ctx->num_cu;
+        if (gB > need) gB = need;
+        if (gB < 1) gB = 1;
+        hipLaunchKernelGGL(k_mixed_ss32<NC>, dim3((unsigned)gB), dim3(block), 0, stream, ctx->d_fluid, b);
+        int pn = 0;
+        OSR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pn, k_mixed_newton<NC>, 64, 0));
+        if (pn < 1) pn = 1;
+        int64_t gN = (int64_t)pn * ctx->num_cu;
+        const int64_t needN = (a.n + 63) / 64;
+        if (gN > needN) gN = needN;
+        if (gN < 1) gN = 1;
+        hipLaunchKernelGGL(k_mixed_newton<NC>, dim3((unsigned)gN), dim3(64), 0, stream, ctx->d_fluid, b);
+    }
+    if (park_after > 0) {
+        constexpr int S = NC <= 8 ? 8 : 16;
+        const int64_t groups_per_block = 256 / (2 * S);
+        int64_t gb = (int64_t)ctx->num_cu * 4;
+        const int64_t maxb = (a.n + groups_per_block - 1) / groups_per_block;
+        if (gb > maxb) gb = maxb;
+        if (gb < 1) gb = 1;
+        hipLaunchKernelGGL((k_flash_group<NC, S>), dim3((unsigned)gb), dim3(256), 0, stream, ctx->d_fluid, b);
+    }
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -283,7 +973,17 @@ int launch_flash(osr_ctx* ctx, int64_t n, const double* T, double T0, const doub
     a.out = *out;
     a.counter = ctx->d_counters;
     a.refill_min = ctx->refill_min < 1 ? 1 : (ctx->refill_min > 64 ? 64 : ctx->refill_min);
-    if (mode == OSR_FLASH_MIXED) mode = OSR_FLASH_PERSISTENT;  // mixed stage kernels: see flash_mixed
+    a.park_after = 0;
+    a.park = nullptr;
+    a.park_cap = 0;
+    a.kb = nullptr;
+    a.itb = nullptr;
+    a.list2 = nullptr;
+    a.list3 = nullptr;
+    a.glist = nullptr;
+    a.glen_slot = CNT_PARK;
+    a.tol32 = ctx->mixed_tol32;
+    a.max32 = ctx->mixed_max32;
     switch (ctx->nc) {
         case 2: return launch_nc<2>(ctx, a, mode, stream);
         case 3: return launch_nc<3>(ctx, a, mode, stream);

@@ -23,7 +23,10 @@ struct osr_ctx {
     // Tuning knobs (set by osr_tune_set; defaults chosen from measurement).
     int flash_block = 256;
     int flash_blocks_per_cu = 0;   // 0 = from occupancy query
-    int refill_min = 16;
+    int refill_min = 48;
+    int park_after = 40;            // persistent flash: passes before a cell moves to the lane-group kernel
+    void* d_park = nullptr;
+    size_t park_bytes = 0;
     int mixed_warm_iters = 2;
     float mixed_tol32 = 1e-5f;
     int mixed_max32 = 200;
@@ -33,6 +36,12 @@ struct osr_ctx {
     do {                                                \
         hipError_t e_ = (call);                         \
         if (e_ != hipSuccess) return OSR_E_HIP;         \
+    } while (0)
+
+#define OSR_HIP_RET(call)                               \
+    do {                                                \
+        const int r_ = (call);                          \
+        if (r_ != OSR_OK) return r_;                    \
     } while (0)
 
 namespace osr {

@@ -217,6 +217,9 @@ def _bind_asm(L):
     L.osr_jacobian_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]
     L.osr_props_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
+    # osr_state_mut_dev has the field layout of osr_state_dev
+    L.osr_unpack_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
+    L.osr_pack_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
     L._asm_bound = True
 
 
@@ -316,6 +319,28 @@ class Assembler:
                                          C.c_void_p(cols.data_ptr()), C.c_void_p(vals.data_ptr()),
                                          _stream_ptr(stream)), "osr_jacobian_device")
         return rowptr, cols, vals
+
+    def pack(self, d, stream=None):
+        """InitGuess2: state -> unknown vector X (device tensor [nrows])."""
+        import torch
+        if self.nrows is None:
+            self.layout(d, stream)
+        X = torch.empty(self.nrows, dtype=torch.float64, device="cuda")
+        sd = self._sd(d)
+        _check(lib().osr_pack_device(self.h, C.byref(sd), C.c_void_p(X.data_ptr()),
+                                     _stream_ptr(stream)), "osr_pack_device")
+        return X
+
+    def unpack(self, d, X, stream=None):
+        """UpdateAll's X -> state step, in place on d's device arrays."""
+        if self.nrows is None:
+            self.layout(d, stream)
+        if X.numel() != self.nrows:
+            raise ValueError(f"X has {X.numel()} entries, layout has {self.nrows} rows")
+        sd = self._sd(d)
+        _check(lib().osr_unpack_device(self.h, C.byref(sd), C.c_void_p(X.data_ptr()),
+                                       _stream_ptr(stream)), "osr_unpack_device")
+        return d
 
     def props(self, d, stream=None):
         import torch

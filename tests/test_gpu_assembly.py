@@ -104,3 +104,47 @@ def test_residual_is_deterministic(gpu, oracle):
     a = _run(gpu, fl, grid, st)
     b = _run(gpu, fl, grid, st)
     assert np.array_equal(a["F"], b["F"]) and np.array_equal(a["vals"], b["vals"])
+
+
+def _perturbed_x(X, off, bs, fl, seed):
+    """Newton-like update of X hitting every UpdateAll branch: small steps,
+    pressure jumps above 1 % (limiter), a jump below the 5 MPa floor,
+    compositions below EPS_COMP and above 1, saturations outside [0, 1]."""
+    rng = np.random.default_rng(seed)
+    X1 = X * (1 + 1e-3 * rng.standard_normal(X.shape))
+    nb = len(bs)
+    blocks = rng.choice(nb, size=max(1, nb // 3), replace=False)
+    for b in blocks:
+        o = off[b]
+        X1[o + rng.integers(0, fl.nc)] = rng.choice([-0.2, 1e-30, 1.7])
+        last = off[b + 1] - 1
+        X1[last] = rng.choice([-0.1, 1.3, 0.5])
+    return X1
+
+
+def test_pack_unpack_parity(gpu, oracle):
+    """InitGuess2 and UpdateAll on the device, bit-exact against the oracle
+    (the same IEEE operations in the same order)."""
+    for name, (fl, grid, st) in _cases(oracle):
+        ctx = gpu.Context(fl)
+        asm = gpu.Assembler(ctx, fl, grid)
+        d = asm.device_state(st)
+        off_d, nrows = asm.layout(d)
+        Xg = asm.pack(d).cpu().numpy()
+        Xo = oracle.pack(fl, grid, st)
+        assert np.array_equal(Xg, Xo), f"{name}: pack"
+        bs, off = oracle.layout(fl, grid, st)
+        X1 = _perturbed_x(Xo, off, bs, fl, seed=5)
+        # pressure unknowns of a few blocks: +50 % (limited) and a drop below 5 MPa
+        nc = fl.nc
+        lab = np.concatenate([st.label_m, st.label_f])
+        pcol = off[:-1] + np.where(lab == 0, 2 * nc, nc)
+        X1[pcol[0]] *= 1.5
+        X1[pcol[-1]] = 4.0e6 / st.pcoeff
+        s1 = oracle.unpack(fl, grid, st, X1)
+        asm.unpack(d, torch.as_tensor(X1).cuda())
+        torch.cuda.synchronize()
+        for k in ("Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f", "S_m", "S_f"):
+            g = d[k].cpu().numpy()
+            r = np.asarray(getattr(s1, k))
+            assert np.array_equal(g, r.reshape(g.shape)), f"{name}: unpack {k}"
