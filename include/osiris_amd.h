@@ -227,10 +227,23 @@ int osr_props_device(osr_asm *a, const osr_state_dev *s, double *out, void *stre
  * autotuner, osiris_amd/tune.py): "flash.refill_min" (1..64: idle lanes a
  * wave waits for before refilling from the work queue), "flash.park_after"
  * (0..500: passes after which the persistent flash hands a cell to the
- * lane-group kernel; 0 = never) and
- * "flash.blocks_per_cu" (0 = occupancy-derived; else 256-thread blocks per
- * CU of the persistent flash grid). */
+ * lane-group kernel; 0 = never), "flash.blocks_per_cu" (0 = occupancy-
+ * derived; else 256-thread blocks per CU of the persistent flash grid), and
+ * for OSR_FLASH_MIXED "flash.mixed_warm_iters" (FP64 passes before the FP32
+ * stage, 1..100), "flash.mixed_tol32" (FP32 stage stop on max |K'/K - 1|) and
+ * "flash.mixed_max32" (FP32 stage pass cap). */
 int osr_tune_set(osr_ctx *ctx, const char *key, double value);
+
+/* Stage counts of the last persistent/mixed flash on this context (no
+ * counterpart in the reference), out[0..nout): [0] cells handed on after the
+ * warm FP64 passes, [1] cells that reached the FP64 Newton polish, [2] cells
+ * finished by FP64 successive substitution from their parked state (the FP32
+ * stage or the Newton declined them), [3] Newton cells accepted, then the
+ * Newton's other exits: [4] L within 1e-3 of 0 or 1, [5] non-finite
+ * residual, [6] singular factorisation, [7] step > 1 or non-finite,
+ * [8] iteration cap, [9] rejected by the final reference pass.  Synchronises
+ * the stream. */
+int osr_flash_stats(osr_ctx *ctx, int64_t *out, int32_t nout, void *stream);
 
 /* Library version string and the component counts compiled in. */
 const char *osr_version(void);

@@ -17,11 +17,12 @@
 // cell properties structure-of-arrays ([field][2n]) so the stencil reads of
 // the residual kernel are coalesced along x.  The residual of a block is the
 // sum of separable pieces -- accumulation, six face fluxes, the matrix-
-// fracture transfer, the fugacity and constraint rows -- and the Jacobian
-// kernel differentiates only the pieces that touch the dependency block it
-// owns (one lane per row block x dependency block x chunk of W unknowns),
-// with forward-mode duals in registers; it writes its entries straight into
-// the CSR arrays, so no atomics and the result is bitwise reproducible.
+// fracture transfer, the fugacity and constraint rows.  The Jacobian takes
+// the property derivatives of every block once (k_dprops) and then
+// differentiates only the pieces that touch each (row block, dependency
+// block) pair, one lane per column, forward mode in registers (k_jacobian);
+// entries go straight into the CSR arrays, so no atomics and the result is
+// bitwise reproducible.
 #include <hipcub/hipcub.hpp>
 
 #include "dual_dev.hpp"
@@ -376,72 +377,120 @@ __global__ void k_take_rho(const double* props, int n, double* rp) {
 }
 
 // ------------------------------------------------------------ geometry
+// Per-cell geometry in scalars only (no arrays indexed by a run-time face
+// number, which would put the struct in scratch memory).
 struct Geo {
-    int ix, iy, iz;
-    int nb[6];          // neighbour cells +x -x +y -y +z -z, -1 outside
-    double dsz[3];      // own sizes
-    double nd[6];       // neighbour sizes along the direction
-    double nh[6];       // neighbour block heights
-    double area[6];     // face areas
+    int c, ix, iy, iz;
+    double ds0, ds1, ds2;   // own sizes
     double h, Vb;
 };
 
 __device__ __forceinline__ Geo geom(const AsmArgs& A, int c) {
     Geo g;
     const int nxy = A.nx * A.ny;
+    g.c = c;
     g.iz = c / nxy;
     const int rem = c - g.iz * nxy;
     g.iy = rem / A.nx;
     g.ix = rem - g.iy * A.nx;
-    g.dsz[0] = A.dx[g.ix];
-    g.dsz[1] = A.dy[g.iy];
-    g.dsz[2] = A.dz[g.iz];
+    g.ds0 = A.dx[g.ix];
+    g.ds1 = A.dy[g.iy];
+    g.ds2 = A.dz[g.iz];
     g.h = A.h[g.iz];
-    g.Vb = g.dsz[0] * g.dsz[1] * g.dsz[2];
-    g.nb[0] = g.ix < A.nx - 1 ? c + 1 : -1;
-    g.nb[1] = g.ix > 0 ? c - 1 : -1;
-    g.nb[2] = g.iy < A.ny - 1 ? c + A.nx : -1;
-    g.nb[3] = g.iy > 0 ? c - A.nx : -1;
-    g.nb[4] = g.iz < A.nz - 1 ? c + nxy : -1;
-    g.nb[5] = g.iz > 0 ? c - nxy : -1;
-    g.nd[0] = g.nb[0] >= 0 ? A.dx[g.ix + 1] : 0;
-    g.nd[1] = g.nb[1] >= 0 ? A.dx[g.ix - 1] : 0;
-    g.nd[2] = g.nb[2] >= 0 ? A.dy[g.iy + 1] : 0;
-    g.nd[3] = g.nb[3] >= 0 ? A.dy[g.iy - 1] : 0;
-    g.nd[4] = g.nb[4] >= 0 ? A.dz[g.iz + 1] : 0;
-    g.nd[5] = g.nb[5] >= 0 ? A.dz[g.iz - 1] : 0;
-    for (int q = 0; q < 4; q++) g.nh[q] = g.h;
-    g.nh[4] = g.nb[4] >= 0 ? A.h[g.iz + 1] : 0;
-    g.nh[5] = g.nb[5] >= 0 ? A.h[g.iz - 1] : 0;
-    g.area[0] = g.area[1] = g.dsz[1] * g.dsz[2];
-    g.area[2] = g.area[3] = g.dsz[0] * g.dsz[2];
-    g.area[4] = g.area[5] = g.dsz[0] * g.dsz[1];
+    g.Vb = g.ds0 * g.ds1 * g.ds2;
     return g;
 }
 
+// neighbour cell across face q (+x -x +y -y +z -z), -1 outside the grid
+__device__ __forceinline__ int nb_of(const AsmArgs& A, const Geo& G, int q) {
+    const int nxy = A.nx * A.ny;
+    switch (q) {
+        case 0: return G.ix < A.nx - 1 ? G.c + 1 : -1;
+        case 1: return G.ix > 0 ? G.c - 1 : -1;
+        case 2: return G.iy < A.ny - 1 ? G.c + A.nx : -1;
+        case 3: return G.iy > 0 ? G.c - A.nx : -1;
+        case 4: return G.iz < A.nz - 1 ? G.c + nxy : -1;
+        default: return G.iz > 0 ? G.c - nxy : -1;
+    }
+}
+
+struct FaceGeo {
+    int o;          // neighbour cell (or -1)
+    double dl, dn;  // own and neighbour size along the face normal
+    double hn;      // neighbour block height (own for x/y faces)
+    double area;
+};
+
+__device__ __forceinline__ FaceGeo face_geo(const AsmArgs& A, const Geo& G, int q) {
+    FaceGeo f;
+    f.o = nb_of(A, G, q);
+    const bool in = f.o >= 0;
+    switch (q >> 1) {
+        case 0:
+            f.dl = G.ds0;
+            f.dn = in ? A.dx[G.ix + ((q & 1) ? -1 : 1)] : 0;
+            f.hn = G.h;
+            f.area = G.ds1 * G.ds2;
+            break;
+        case 1:
+            f.dl = G.ds1;
+            f.dn = in ? A.dy[G.iy + ((q & 1) ? -1 : 1)] : 0;
+            f.hn = G.h;
+            f.area = G.ds0 * G.ds2;
+            break;
+        default:
+            f.dl = G.ds2;
+            f.dn = in ? A.dz[G.iz + ((q & 1) ? -1 : 1)] : 0;
+            f.hn = in ? A.h[G.iz + ((q & 1) ? -1 : 1)] : 0;
+            f.area = G.ds0 * G.ds1;
+            break;
+    }
+    return f;
+}
+
 // ------------------------------------------------------- residual pieces
-// All pieces add into rows[] (the block's residual, F = LSide - RSide, so
-// inflow terms are subtracted).  r0: first component-balance row.
+// All pieces add into a RowAcc (the block's residual, F = LSide - RSide, so
+// inflow terms are subtracted), indexed statically so it stays in registers:
+//   bal[0..NC-2] component balances, bal[NC-1] total hydrocarbon, bal[NC]
+//   water -- reference rows r0 + i (r0 = NC two-phase, 0 single-phase);
+//   loc[] the block-local rows -- two-phase: fugacity i -> row i, then
+//   sum x, sum y, sum S -> rows 2NC+1..2NC+3; single-phase: sum x|y, sum S
+//   -> rows NC+1, NC+2 (IteratorFunction's row order).
+template <int NC, class T>
+struct RowAcc {
+    T bal[NC + 1];
+    T loc[NC + 3];
+    __device__ __forceinline__ RowAcc() {
+#pragma unroll
+        for (int i = 0; i <= NC; i++) bal[i] = T(0.0);
+#pragma unroll
+        for (int i = 0; i < NC + 3; i++) loc[i] = T(0.0);
+    }
+};
+
+// reference row of local row i
+__device__ __forceinline__ int loc_row(int lab, int nc, int i) { return (lab == 0 && i < nc) ? i : nc + 1 + i; }
+__device__ __forceinline__ int nloc(int lab, int nc) { return lab == 0 ? nc + 3 : 2; }
 
 // Flux across face q between self (s) and neighbour (o): convective and
 // gravity terms of every component, total and water row, and the Fickian
 // flux of each component (Calc_Transes + IteratorFunction face terms).
 template <int NC, class TS, class TO>
-__device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G, int q, int c, int o,
-                                           const VarsT<NC, TS>& vs, const PropsT<NC, TS>& ps,
-                                           const VarsT<NC, TO>& vo, const PropsT<NC, TO>& po, int r0,
-                                           decltype(TS() + TO())* rows) {
+__device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G, const FaceGeo& FG, int q, int c,
+                                           int o, const VarsT<NC, TS>& vs, const PropsT<NC, TS>& ps,
+                                           const VarsT<NC, TO>& vo, const PropsT<NC, TO>& po,
+                                           RowAcc<NC, decltype(TS() + TO())>& R) {
     using T = decltype(TS() + TO());
     const AsmConsts& K = *A.K;
     const int d = q >> 1;
     const bool self_is_a = (q & 1) == 0;
     const int ca = self_is_a ? c : o;
     const double Kf = A.iftran[((int64_t)k * 3 + d) * A.n + ca];
-    const double da = self_is_a ? G.dsz[d] : G.nd[q];
-    const double db = self_is_a ? G.nd[q] : G.dsz[d];
-    const double ha = self_is_a ? G.h : G.nh[q];
-    const double hb = self_is_a ? G.nh[q] : G.h;
-    const double dh = G.nh[q] - G.h;
+    const double da = self_is_a ? FG.dl : FG.dn;
+    const double db = self_is_a ? FG.dn : FG.dl;
+    const double ha = self_is_a ? G.h : FG.hn;
+    const double hb = self_is_a ? FG.hn : G.h;
+    const double dh = FG.hn - G.h;
     // water (upstream by water potential)
     {
         const T pot = (self_is_a ? (ps.Pw - po.Pw) : (po.Pw - ps.Pw)) + G_ACC * K.wat_ro * (ha - hb);
@@ -449,7 +498,7 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
         const bool up_self = (up_a == self_is_a);
         const T tw = (Kf * (K.wat_ro * WAT_M_RO / K.wat_mu)) * (up_self ? T(ps.kr[0]) : T(po.kr[0]));
         const T fw = tw * (po.Pw - ps.Pw) + G_ACC * K.wat_ro * tw * dh;
-        rows[r0 + NC] -= fw;
+        R.bal[NC] -= fw;
     }
     // oil and gas
 #pragma unroll
@@ -483,17 +532,17 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
         const T dPp = Poth - Psel;
         const T base = t * dPp + tg * dh;
         // total hydrocarbon row
-        rows[r0 + NC - 1] -= base;
+        R.bal[NC - 1] -= base;
 #pragma unroll
         for (int i = 0; i < NC - 1; i++) {
             const T cu = up_self ? (ph == 0 ? T(vs.x[i]): T(vs.y[i]))
                                  : (ph == 0 ? T(vo.x[i]): T(vo.y[i]));
-            rows[r0 + i] -= cu * base;
+            R.bal[i] -= cu * base;
         }
     }
     // Fickian inter-cell flux of the components (tortuosity 1 in the fracture)
-    const double dl = G.dsz[d], dn = G.nd[q];
-    const double NN = 2.0 * G.area[q] / (dn + dl);
+    const double dl = FG.dl, dn = FG.dn;
+    const double NN = 2.0 * FG.area / (dn + dl);
     const double tn = k == 0 ? A.tor[o] : 1.0, ts = k == 0 ? A.tor[c] : 1.0;
     const bool zerophi = A.phi[k * A.n + c] < 1e-10 || A.phi[k * A.n + o] < 1e-10;
 #pragma unroll
@@ -513,7 +562,7 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
             if (zerophi) Dup = T(0.0);
             T Dt = MN * Dup;
             if (Cf == 0 && dcs == 0) Dt = T(0.0);
-            rows[r0 + i] -= Dt * (xn - xs);
+            R.bal[i] -= Dt * (xn - xs);
         }
     }
 }
@@ -523,13 +572,13 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
 template <int NC, class TM, class TF>
 __device__ __forceinline__ void transfer_piece(const AsmArgs& A, int k, const Geo& G, int c,
                                                const VarsT<NC, TM>& vm, const PropsT<NC, TM>& pm,
-                                               const VarsT<NC, TF>& vf, const PropsT<NC, TF>& pf, int r0,
-                                               decltype(TM() + TF())* rows) {
+                                               const VarsT<NC, TF>& vf, const PropsT<NC, TF>& pf,
+                                               RowAcc<NC, decltype(TM() + TF())>& R) {
     using T = decltype(TM() + TF());
     const AsmConsts& K = *A.K;
     const int n = A.n;
     const double Vb = G.Vb;
-    const double dx = G.dsz[0], dy = G.dsz[1], dz = G.dsz[2];
+    const double dx = G.ds0, dy = G.ds1, dz = G.ds2;
     const double sf2 = 4 * K.sigma * (A.perm_m[c] / (dx * dx) + A.perm_m[n + c] / (dy * dy) +
                                       A.perm_m[2 * n + c] / (dz * dz));
     const bool off = K.const_diff && K.cdiff == 0;
@@ -544,9 +593,9 @@ __device__ __forceinline__ void transfer_piece(const AsmArgs& A, int k, const Ge
     const T mug = (pm.mu[1] * pm.porpor + pf.mu[1] * pf.porpor) / sp;
     const T convo = (SI_C * sf2 * Vb) * pm.ro[0] * kro / muo * (pf.Po - pm.Po);
     const T convg = (SI_C * sf2 * Vb) * pm.ro[1] * krg / mug * ((pf.Pg - pm.Pg) + (pf.pcg - pm.pcg));
-    rows[r0 + NC - 1] -= sgn * (convo + convg);
+    R.bal[NC - 1] -= sgn * (convo + convg);
     const T convw = (SI_C * K.wat_ro * WAT_M_RO * sf2 * Vb / K.wat_mu) * krw * ((pf.Pw - pm.Pw) + (pm.pcw - pf.pcw));
-    rows[r0 + NC] -= sgn * convw;
+    R.bal[NC] -= sgn * convw;
     const int lm = vm.lab, lf = vf.lab;
     const double tor = A.tor[c];
     const T mo0 = (sf1 * Vb / tor) * pm.porpor * vm.S[1] * pm.ro[0];
@@ -582,7 +631,7 @@ __device__ __forceinline__ void transfer_piece(const AsmArgs& A, int k, const Ge
             if (lm == 1 && lf == 0) { rs -= fg * fyo; rs -= mo * dxo; }
             if (lm == -1 && lf == 0) { rs -= mg * dyo; rs -= fo * fxo; }
         }
-        rows[r0 + i] -= rs;
+        R.bal[i] -= rs;
     }
 }
 
@@ -591,7 +640,7 @@ __device__ __forceinline__ void transfer_piece(const AsmArgs& A, int k, const Ge
 // unknowns only.
 template <int NC, class T>
 __device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const Geo& G, const VarsT<NC, T>& v,
-                                          const PropsT<NC, T>& p, int r0, T* rows) {
+                                          const PropsT<NC, T>& p, RowAcc<NC, T>& R) {
     const AsmConsts& K = *A.K;
     const int n = A.n;
     const double f = G.Vb / K.dt;
@@ -602,12 +651,12 @@ __device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const 
         if (i < NC - 1) acc = p.porpor * (v.x[i] * p.ro[0] * v.S[1] + v.y[i] * p.ro[1] * v.S[2]);
         else acc = p.porpor * (p.ro[0] * v.S[1] + p.ro[1] * v.S[2]);
         const double pr = pre ? pre[(int64_t)i * n + c] : dval(acc);
-        rows[r0 + i] += f * (acc - pr);
+        R.bal[i] += f * (acc - pr);
     }
     {
         const T acc = p.porpor * (K.wat_ro * WAT_M_RO) * v.S[0];
         const double pr = pre ? pre[(int64_t)NC * n + c] : dval(acc);
-        rows[r0 + NC] += f * (acc - pr);
+        R.bal[NC] += f * (acc - pr);
     }
     if (v.lab == 0) {
         T Cal, Cbl, Cag, Cbg, tl[NC], tg[NC];
@@ -627,7 +676,7 @@ __device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const 
             const T EEl = 2.0 * tl[i] / Cal - t1l, EEg = 2.0 * tg[i] / Cag - t1g;
             const T phiL = fexp(t1l * (Zl - 1.0) - lzl + DDl * EEl * FFl);
             const T phiG = fexp(t1g * (Zg - 1.0) - lzg + DDg * EEg * FFg);
-            rows[i] += v.x[i] * p.Po * phiL - v.y[i] * p.Pg * phiG;
+            R.loc[i] += v.x[i] * p.Po * phiL - v.y[i] * p.Pg * phiG;
         }
         T sx = 0.0, sy = 0.0;
 #pragma unroll
@@ -635,15 +684,15 @@ __device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const 
             sx += v.x[i];
             sy += v.y[i];
         }
-        rows[2 * NC + 1] += sx - 1.0;
-        rows[2 * NC + 2] += sy - 1.0;
-        rows[2 * NC + 3] += v.S[0] + v.S[1] + v.S[2] - 1.0;
+        R.loc[NC] += sx - 1.0;
+        R.loc[NC + 1] += sy - 1.0;
+        R.loc[NC + 2] += v.S[0] + v.S[1] + v.S[2] - 1.0;
     } else {
         T s = 0.0;
 #pragma unroll
         for (int i = 0; i < NC; i++) s += (v.lab == 1 ? v.x[i] : v.y[i]);
-        rows[NC + 1] += s - 1.0;
-        rows[NC + 2] += v.S[0] + v.S[1] + v.S[2] - 1.0;
+        R.loc[0] += s - 1.0;
+        R.loc[1] += v.S[0] + v.S[1] + v.S[2] - 1.0;
     }
 }
 
@@ -663,31 +712,36 @@ __global__ __launch_bounds__(256) void k_residual(AsmArgs A, double* F) {
         return;
     }
     const int r0 = v.lab == 0 ? NC : 0;
-    double rows[2 * NC + 4];
-#pragma unroll
-    for (int r = 0; r < 2 * NC + 4; r++) rows[r] = 0;
+    RowAcc<NC, double> R;
     const Geo G = geom(A, c);
     PropsT<NC, double> p;
     load_props<NC>(A, k, c, p);
-    own_piece<NC, double>(A, k, c, G, v, p, r0, rows);
+    own_piece<NC, double>(A, k, c, G, v, p, R);
+#pragma unroll
     for (int q = 0; q < 6; q++) {
-        const int o = G.nb[q];
+        const FaceGeo FG = face_geo(A, G, q);
+        const int o = FG.o;
         if (o < 0) continue;
         VarsT<NC, double> vo;
         PropsT<NC, double> po;
         load_vars<NC>(A, k, o, vo);
         load_props<NC>(A, k, o, po);
-        face_piece<NC, double, double>(A, k, G, q, c, o, v, p, vo, po, r0, rows);
+        face_piece<NC, double, double>(A, k, G, FG, q, c, o, v, p, vo, po, R);
     }
     {
         VarsT<NC, double> vx;
         PropsT<NC, double> px;
         load_vars<NC>(A, 1 - k, c, vx);
         load_props<NC>(A, 1 - k, c, px);
-        if (k == 0) transfer_piece<NC, double, double>(A, 0, G, c, v, p, vx, px, r0, rows);
-        else transfer_piece<NC, double, double>(A, 1, G, c, vx, px, v, p, r0, rows);
+        if (k == 0) transfer_piece<NC, double, double>(A, 0, G, c, v, p, vx, px, R);
+        else transfer_piece<NC, double, double>(A, 1, G, c, vx, px, v, p, R);
     }
-    for (int r = 0; r < bs; r++) out[r] = rows[r];
+#pragma unroll
+    for (int i = 0; i <= NC; i++) out[r0 + i] = R.bal[i];
+    const int nl = nloc(v.lab, NC);
+#pragma unroll
+    for (int i = 0; i < NC + 3; i++)
+        if (i < nl) out[loc_row(v.lab, NC, i)] = R.loc[i];
 }
 
 // ------------------------------------------------------ Jacobian pattern
@@ -699,7 +753,7 @@ __global__ __launch_bounds__(256) void k_residual(AsmArgs A, double* F) {
 __device__ __forceinline__ int dep_block(const AsmArgs& A, int k, const Geo& G, int c, int slot) {
     if (slot == 0) return k * A.n + c;
     if (slot == 7) return (1 - k) * A.n + c;
-    const int o = G.nb[slot - 1];
+    const int o = nb_of(A, G, slot - 1);
     return o < 0 ? -1 : k * A.n + o;
 }
 
@@ -784,108 +838,232 @@ __device__ __forceinline__ void seed_vars(const VarsT<NC, double>& v, int chunk,
     if (v.lab != 1) mark(out.S[2], 1.0);
 }
 
-template <int NC, int W>
-__global__ __launch_bounds__(64) void k_jacobian(AsmArgs A, const int64_t* rowptr, int32_t* cols, double* vals,
-                                                 int nchunk) {
+// Jacobian in two kernels.  Every residual piece of a row block depends on a
+// dependency block d through d's unknowns directly and through d's cell
+// properties, and the properties of d are shared by the eight row blocks
+// that touch d.  So:
+//   k_dprops    one lane per (block d, unknown j): the property sweep of d in
+//               forward mode with unknown j seeded (one direction per lane,
+//               DualV<1>, everything in registers); writes d props / d u_j to
+//               a workspace laid out [block][field][column];
+//   k_jacobian  one lane per (row block, dependency slot, column j): the
+//               pieces that touch the dependency, in DualV<1>, with the
+//               dependency's properties rebuilt from the property table
+//               (value) and the workspace (derivative).  The lanes of one
+//               (row block, dependency) write consecutive CSR columns.
+template <int NC>
+struct JacShape {
+    static constexpr int NF = 16 + 2 * NC;
+    static constexpr int MAXC = 2 * NC + 4;
+};
+
+template <int NC, class T>
+__device__ __forceinline__ T prop_field(const PropsT<NC, T>& p, int f) {
+    switch (f) {
+        case 0: return p.porpor;
+        case 1: return p.Pw;
+        case 2: return p.Po;
+        case 3: return p.Pg;
+        case 4: return p.bift;
+        case 5: return p.ro[0];
+        case 6: return p.ro[1];
+        case 7: return p.mu[0];
+        case 8: return p.mu[1];
+        case 9: return p.gam[0];
+        case 10: return p.gam[1];
+        case 11: return p.kr[0];
+        case 12: return p.kr[1];
+        case 13: return p.kr[2];
+        case 14: return p.pcw;
+        case 15: return p.pcg;
+        default: return f < 16 + NC ? p.D[0][f - 16] : p.D[1][f - 16 - NC];
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_dprops(AsmArgs A, double* dp) {
+    using D1 = DualV<1>;
+    constexpr int NF = JacShape<NC>::NF, MAXC = JacShape<NC>::MAXC;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = (int64_t)2 * A.n * 8 * nchunk;
-    if (item >= total) return;
-    const int chunk = (int)(item % nchunk);
-    const int slot = (int)((item / nchunk) % 8);
-    const int blk = (int)(item / (8 * nchunk));
+    if (item >= (int64_t)2 * A.n * MAXC) return;
+    const int blk = (int)(item / MAXC);
+    const int j = (int)(item % MAXC);
     const int n = A.n, k = blk / n, c = blk % n;
+    VarsT<NC, double> vd;
+    load_vars<NC>(A, k, c, vd);
+    const int bs = vd.lab == 0 ? 2 * NC + 4 : NC + 3;
+    if (j >= bs) return;
+    VarsT<NC, D1> sv;
+    seed_vars<NC, 1>(vd, j, A.pcoeff, sv);
+    double rpo = 0, rpg = 0;
+    if (k == 0 && A.rho_prev) {
+        rpo = A.rho_prev[c];
+        rpg = A.rho_prev[n + c];
+    }
+    PropsT<NC, D1> sp;
+    propsT<NC, D1>(A, k, c, sv.lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp);
+    double* o = dp + (int64_t)blk * NF * MAXC + j;
+#pragma unroll
+    for (int f = 0; f < NF; f++) o[f * MAXC] = prop_field<NC, D1>(sp, f).d[0];
+}
+
+template <int NC>
+__device__ __forceinline__ void load_props_dual(const AsmArgs& A, const double* dp, int blk, int j,
+                                                PropsT<NC, DualV<1>>& p) {
+    constexpr int NF = JacShape<NC>::NF, MAXC = JacShape<NC>::MAXC;
+    const int64_t s = 2 * (int64_t)A.n;
+    const double* q = A.props + blk;
+    const double* g = dp + (int64_t)blk * NF * MAXC + j;
+    auto F = [&](int f) {
+        DualV<1> r;
+        r.v = q[f * s];
+        r.d[0] = g[f * MAXC];
+        return r;
+    };
+    p.porpor = F(0); p.Pw = F(1); p.Po = F(2); p.Pg = F(3); p.bift = F(4);
+    p.ro[0] = F(5); p.ro[1] = F(6); p.mu[0] = F(7); p.mu[1] = F(8);
+    p.gam[0] = F(9); p.gam[1] = F(10);
+    p.kr[0] = F(11); p.kr[1] = F(12); p.kr[2] = F(13); p.pcw = F(14); p.pcg = F(15);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        p.D[0][i] = F(16 + i);
+        p.D[1][i] = F(16 + NC + i);
+    }
+}
+
+// Column j of dependency d, for the rows of block blk, into the CSR.  Mass
+// rows list every dependency in block-offset order, the water row lists its
+// own block whole and only P, Sw of the others, local rows their own block.
+template <int NC>
+__device__ __forceinline__ void scatter_col(const AsmArgs& A, const Geo& G, int k, int c, int blk, int lab,
+                                            int slot, int d, int j, const RowAcc<NC, DualV<1>>& R,
+                                            const int64_t* rowptr, int32_t* cols, double* vals) {
+    const int n = A.n;
+    const int r0 = lab == 0 ? NC : 0;
+    const int64_t rb = A.boff[blk];
+    const int32_t col = (int32_t)(A.boff[d] + j);
+    const int64_t s_mass = seg_start(A, k, G, c, 0, d, NC);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        const int64_t pos = rowptr[rb + r0 + i] + s_mass + j;
+        cols[pos] = col;
+        vals[pos] = R.bal[i].d[0];
+    }
+    const int64_t s_water = seg_start(A, k, G, c, 1, d, NC);
+    if (slot == 0) {
+        const int64_t pos = rowptr[rb + r0 + NC] + s_water + j;
+        cols[pos] = col;
+        vals[pos] = R.bal[NC].d[0];
+        const int nl = nloc(lab, NC);
+#pragma unroll
+        for (int i = 0; i < NC + 3; i++) {
+            if (i >= nl) break;
+            const int64_t p2 = rowptr[rb + loc_row(lab, NC, i)] + j;
+            cols[p2] = col;
+            vals[p2] = R.loc[i].d[0];
+        }
+    } else {
+        const int dk = d / n, dc = d % n;
+        const int pcol = (A.lab[dk][dc] == 0) ? 2 * NC : NC;
+        if (j == pcol || j == pcol + 1) {
+            const int64_t pos = rowptr[rb + r0 + NC] + s_water + (j - pcol);
+            cols[pos] = col;
+            vals[pos] = R.bal[NC].d[0];
+        }
+    }
+}
+
+// own block (slot 0): accumulation, fugacity and constraint rows, the six
+// faces and the transfer, all with the block's own unknown j seeded
+template <int NC>
+__global__ __launch_bounds__(256) void k_jac_own(AsmArgs A, const double* dp, const int64_t* rowptr,
+                                                 int32_t* cols, double* vals) {
+    using D1 = DualV<1>;
+    constexpr int MAXC = JacShape<NC>::MAXC;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (int64_t)2 * A.n * MAXC) return;
+    const int j = (int)(item % MAXC);
+    const int blk = (int)(item / MAXC);
+    const int n = A.n, k = blk / n, c = blk % n;
+    const int lab = A.lab[k][c];
+    const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
+    if (j >= bs) return;
+    if (A.phi[k * n + c] < 1e-10) {
+        // zero-porosity block: identity rows (Petsc_Functions.cpp:42051-42056)
+        const int64_t row = A.boff[blk] + j;
+        cols[rowptr[row]] = (int32_t)row;
+        vals[rowptr[row]] = 1.0;
+        return;
+    }
+    const Geo G = geom(A, c);
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    VarsT<NC, D1> sv;
+    seed_vars<NC, 1>(v, j, A.pcoeff, sv);
+    PropsT<NC, D1> sp;
+    load_props_dual<NC>(A, dp, blk, j, sp);
+    RowAcc<NC, D1> R;
+    own_piece<NC, D1>(A, k, c, G, sv, sp, R);
+    for (int q = 0; q < 6; q++) {
+        const FaceGeo FG = face_geo(A, G, q);
+        const int o = FG.o;
+        if (o < 0) continue;
+        VarsT<NC, double> vo;
+        PropsT<NC, double> po;
+        load_vars<NC>(A, k, o, vo);
+        load_props<NC>(A, k, o, po);
+        face_piece<NC, D1, double>(A, k, G, FG, q, c, o, sv, sp, vo, po, R);
+    }
+    {
+        VarsT<NC, double> vx;
+        PropsT<NC, double> px;
+        load_vars<NC>(A, 1 - k, c, vx);
+        load_props<NC>(A, 1 - k, c, px);
+        if (k == 0) transfer_piece<NC, D1, double>(A, 0, G, c, sv, sp, vx, px, R);
+        else transfer_piece<NC, double, D1>(A, 1, G, c, vx, px, sv, sp, R);
+    }
+    scatter_col<NC>(A, G, k, c, blk, lab, 0, blk, j, R, rowptr, cols, vals);
+}
+
+// dependency slots 1..7: one face (neighbour seeded) or the transfer (the
+// other continuum seeded); only the balance and water rows depend on them
+template <int NC>
+__global__ __launch_bounds__(256) void k_jac_nb(AsmArgs A, const double* dp, const int64_t* rowptr,
+                                                int32_t* cols, double* vals) {
+    using D1 = DualV<1>;
+    constexpr int MAXC = JacShape<NC>::MAXC;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (int64_t)2 * A.n * 7 * MAXC) return;
+    const int j = (int)(item % MAXC);
+    const int slot = 1 + (int)((item / MAXC) % 7);
+    const int blk = (int)(item / (7 * MAXC));
+    const int n = A.n, k = blk / n, c = blk % n;
+    if (A.phi[k * n + c] < 1e-10) return;
     const Geo G = geom(A, c);
     const int d = dep_block(A, k, G, c, slot);
     if (d < 0) return;
     const int dk = d / n, dc = d % n;
-    const int dbs = bsize_of(A, d, NC);
-    if (chunk * W >= dbs) return;
+    if (j >= bsize_of(A, d, NC)) return;
+    const int lab = A.lab[k][c];
     VarsT<NC, double> v;
     load_vars<NC>(A, k, c, v);
-    const int lab = v.lab;
-    const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
-    const int r0 = lab == 0 ? NC : 0;
-    using DW = DualV<W>;
-    DW rows[2 * NC + 4];
-    if (A.phi[k * n + c] < 1e-10) {
-        // zero-porosity block: identity rows (Petsc_Functions.cpp:42051-42056)
-        if (slot == 0 && chunk == 0)
-            for (int r = 0; r < bs; r++) {
-                const int64_t row = A.boff[blk] + r;
-                cols[rowptr[row]] = (int32_t)row;
-                vals[rowptr[row]] = 1.0;
-            }
-        return;
+    PropsT<NC, double> p;
+    load_props<NC>(A, k, c, p);
+    VarsT<NC, double> vd;
+    load_vars<NC>(A, dk, dc, vd);
+    VarsT<NC, D1> sv;
+    seed_vars<NC, 1>(vd, j, A.pcoeff, sv);
+    PropsT<NC, D1> sp;
+    load_props_dual<NC>(A, dp, d, j, sp);
+    RowAcc<NC, D1> R;
+    if (slot == 7) {
+        if (k == 0) transfer_piece<NC, double, D1>(A, 0, G, c, v, p, sv, sp, R);
+        else transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, v, p, R);
+    } else {
+        const FaceGeo FG = face_geo(A, G, slot - 1);
+        face_piece<NC, double, D1>(A, k, G, FG, slot - 1, c, dc, v, p, sv, sp, R);
     }
-    {
-        VarsT<NC, double> vd;
-        load_vars<NC>(A, dk, dc, vd);
-        VarsT<NC, DW> sv;
-        seed_vars<NC, W>(vd, chunk, A.pcoeff, sv);
-        double rpo = 0, rpg = 0;
-        if (dk == 0) {
-            if (A.rho_prev) {
-                rpo = A.rho_prev[dc];
-                rpg = A.rho_prev[n + dc];
-            }
-        }
-        // previous densities come from the [2n] buffer (set up by the host)
-        PropsT<NC, DW> sp;
-        propsT<NC, DW>(A, dk, dc, sv.lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp);
-        if (slot == 0) {
-            own_piece<NC, DW>(A, k, c, G, sv, sp, r0, rows);
-            for (int q = 0; q < 6; q++) {
-                const int o = G.nb[q];
-                if (o < 0) continue;
-                VarsT<NC, double> vo;
-                PropsT<NC, double> po;
-                load_vars<NC>(A, k, o, vo);
-                load_props<NC>(A, k, o, po);
-                face_piece<NC, DW, double>(A, k, G, q, c, o, sv, sp, vo, po, r0, rows);
-            }
-            VarsT<NC, double> vx;
-            PropsT<NC, double> px;
-            load_vars<NC>(A, 1 - k, c, vx);
-            load_props<NC>(A, 1 - k, c, px);
-            if (k == 0) transfer_piece<NC, DW, double>(A, 0, G, c, sv, sp, vx, px, r0, rows);
-            else transfer_piece<NC, double, DW>(A, 1, G, c, vx, px, sv, sp, r0, rows);
-        } else if (slot == 7) {
-            PropsT<NC, double> p;
-            load_props<NC>(A, k, c, p);
-            if (k == 0) transfer_piece<NC, double, DW>(A, 0, G, c, v, p, sv, sp, r0, rows);
-            else transfer_piece<NC, DW, double>(A, 1, G, c, sv, sp, v, p, r0, rows);
-        } else {
-            PropsT<NC, double> p;
-            load_props<NC>(A, k, c, p);
-            face_piece<NC, double, DW>(A, k, G, slot - 1, c, dc, v, p, sv, sp, r0, rows);
-        }
-    }
-    // scatter into CSR
-    const int pcol = (A.lab[dk][dc] == 0) ? 2 * NC : NC;
-    const int64_t rb = A.boff[blk];
-    const int64_t dboff = A.boff[d];
-    for (int r = 0; r < bs; r++) {
-        const bool mass = r >= r0 && r < r0 + NC;
-        const bool water = r == r0 + NC;
-        if (slot != 0 && !mass && !water) continue;
-        const int kind = mass ? 0 : (water ? 1 : 2);
-        const int64_t start = rowptr[rb + r] + seg_start(A, k, G, c, kind, d, NC);
-#pragma unroll
-        for (int w = 0; w < W; w++) {
-            const int j = chunk * W + w;
-            if (j >= dbs) break;
-            if (water && slot != 0) {
-                if (j != pcol && j != pcol + 1) continue;
-                const int64_t pos = start + (j - pcol);
-                cols[pos] = (int32_t)(dboff + j);
-                vals[pos] = rows[r].d[w];
-            } else {
-                const int64_t pos = start + j;
-                cols[pos] = (int32_t)(dboff + j);
-                vals[pos] = rows[r].d[w];
-            }
-        }
-    }
+    scatter_col<NC>(A, G, k, c, blk, lab, slot, d, j, R, rowptr, cols, vals);
 }
 
 // block sizes of the layout
@@ -912,6 +1090,7 @@ struct osr_asm {
     double* rho_tmp = nullptr;     // [2n]
     int64_t* boff = nullptr;       // [2n+1]
     int64_t* bs = nullptr;         // [2n]
+    double* dprops = nullptr;      // Jacobian workspace [2n][NF][2nc+4]
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     int64_t nrows = -1;
@@ -1007,11 +1186,17 @@ int jacobian_nc(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, int32
     osr::AsmArgs A = make_args(a, s);
     int rc = run_props<NC>(a, A, st);
     if (rc) return rc;
-    constexpr int W = (2 * NC + 4) <= 16 ? (2 * NC + 4) : 8;
-    const int nchunk = (2 * NC + 4 + W - 1) / W;
-    const int64_t total = (int64_t)2 * a->n * 8 * nchunk;
-    const int64_t grid = (total + 63) / 64;
-    hipLaunchKernelGGL((osr::k_jacobian<NC, W>), dim3((unsigned)grid), dim3(64), 0, st, A, rowptr, cols, vals, nchunk);
+    constexpr int NF = osr::JacShape<NC>::NF, MAXC = osr::JacShape<NC>::MAXC;
+    const int64_t nb = 2 * (int64_t)a->n;
+    if (!a->dprops) OSR_HIP(hipMalloc((void**)&a->dprops, (size_t)nb * NF * MAXC * sizeof(double)));
+    const int64_t ga = (nb * MAXC + 255) / 256;
+    hipLaunchKernelGGL(osr::k_dprops<NC>, dim3((unsigned)ga), dim3(256), 0, st, A, a->dprops);
+    const int64_t go = (nb * MAXC + 255) / 256;
+    hipLaunchKernelGGL(osr::k_jac_own<NC>, dim3((unsigned)go), dim3(256), 0, st, A, (const double*)a->dprops,
+                       rowptr, cols, vals);
+    const int64_t gn = (nb * 7 * MAXC + 255) / 256;
+    hipLaunchKernelGGL(osr::k_jac_nb<NC>, dim3((unsigned)gn), dim3(256), 0, st, A, (const double*)a->dprops,
+                       rowptr, cols, vals);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -1155,7 +1340,7 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
 int osr_asm_destroy(osr_asm* a) {
     if (!a) return OSR_E_ARG;
     void* ptrs[] = {a->dK, a->dx, a->dy, a->dz, a->h, a->phi, a->perm_m, a->iftran, a->tor,
-                    a->props, a->rho_tmp, a->boff, a->bs, a->scan_tmp};
+                    a->props, a->rho_tmp, a->boff, a->bs, a->dprops, a->scan_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete a;

@@ -132,6 +132,20 @@ int osr_tune_set(osr_ctx* ctx, const char* key, double value) {
     return OSR_OK;
 }
 
+int osr_flash_stats(osr_ctx* ctx, int64_t* out, int32_t nout, void* stream) {
+    if (!ctx || (!out && nout > 0) || nout < 0) return OSR_E_ARG;
+    unsigned long long h[16];
+    hipStream_t st = (hipStream_t)stream;
+    OSR_HIP(hipMemcpyAsync(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost, st));
+    OSR_HIP(hipStreamSynchronize(st));
+    // parked, newton, fallback, newton accepted, then the Newton exits:
+    // boundary, non-finite, singular, diverged, iteration cap, final-pass reject
+    const int slot[10] = {1, 4, 6, 7, 8, 9, 10, 11, 12, 13};
+    for (int i = 0; i < nout && i < 10; i++) out[i] = (int64_t)h[slot[i]];
+    for (int i = 10; i < nout; i++) out[i] = 0;
+    return OSR_OK;
+}
+
 int osr_phase_ctrl_device(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* label,
                           const double* Po, const double* Pg, double* x, double* y, double* S,
                           double* psat, void* stream) {

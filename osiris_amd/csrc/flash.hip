@@ -49,6 +49,9 @@ struct FlashArgs {
 // counter slots
 constexpr int CNT_WORK = 0, CNT_PARK = 1, CNT_GROUP = 2, CNT_B = 3, CNT_NT_TAIL = 4, CNT_NT_HEAD = 5,
               CNT_FB_TAIL = 6;
+// Newton-stage outcomes (telemetry, read by osr_flash_stats)
+constexpr int CNT_NT_OK = 7, CNT_NT_BOUNDARY = 8, CNT_NT_NONFINITE = 9, CNT_NT_SINGULAR = 10,
+              CNT_NT_DIVERGED = 11, CNT_NT_MAXIT = 12, CNT_NT_REJECT = 13;
 constexpr int NCOUNTERS = 16;
 
 template <int NC, class R>
@@ -659,6 +662,7 @@ __device__ __forceinline__ void add_phase_block(const FluidConsts<float>& Cf, co
 }
 
 constexpr int MIXED_NEWTON_MAX = 8;
+constexpr double MIXED_ACCEPT = 1e-10;   // max |ln K'/K| of the final reference pass
 
 // Stage C.  One lane per cell, per-lane work loop (no cross-lane operation);
 // J [NC][NC+1] (augmented with -g) lives in LDS as FP32, column-interleaved
@@ -695,6 +699,7 @@ void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
         const double pB = P / rt, pA = pB / rt;
         int rr = (int)a.itb[cap + q];
         bool ok = false;
+        int why = CNT_NT_MAXIT;
         int itn = 0;
         for (; itn < MIXED_NEWTON_MAX; itn++) {
             // Rachford-Rice exactly as the reference pass, with dL/dlnK carried along
@@ -728,7 +733,10 @@ void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
                 j++;
             } while (step * step > NEWTON_TOL && j < MAX_FLASH_N_ITR);
             rr += j;
-            if (!(L > 1e-3 && L < 1.0 - 1e-3)) break;   // near or past a boundary: FP64 SS decides
+            if (!(L > 1e-3 && L < 1.0 - 1e-3)) {   // near or past a boundary: FP64 SS decides
+                why = CNT_NT_BOUNDARY;
+                break;
+            }
             double x[NC], g[NC];
             float dd[NC], uu[NC], w[NC];
 #pragma unroll
@@ -778,7 +786,10 @@ void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
                 }
                 JM(i, NC) = (float)(-gi);
             }
-            if (!fin) break;
+            if (!fin) {
+                why = CNT_NT_NONFINITE;
+                break;
+            }
             // LU with partial pivoting on the augmented [J | -g]
             bool sing = false;
 #pragma unroll
@@ -807,7 +818,10 @@ void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
                     for (int cc = k + 1; cc <= NC; cc++) JM(rw, cc) -= f * JM(k, cc);
                 }
             }
-            if (sing) break;
+            if (sing) {
+                why = CNT_NT_SINGULAR;
+                break;
+            }
             float dlt[NC];
 #pragma unroll
             for (int k = NC - 1; k >= 0; k--) {
@@ -824,7 +838,10 @@ void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
                     dmax = fmax(dmax, fabs((double)dlt[i]));
                 }
             }
-            if (!isfinite(dmax) || dmax > 1.0) break;
+            if (!isfinite(dmax) || dmax > 1.0) {
+                why = CNT_NT_DIVERGED;
+                break;
+            }
             if (dmax < 1e-9) {
                 ok = true;
                 itn++;
@@ -833,6 +850,7 @@ void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
         }
 #undef JM
         if (!ok) {
+            atomicAdd(a.counter + why, 1ull);
             const unsigned long long p3 = atomicAdd(a.counter + CNT_FB_TAIL, 1ull);
             a.list3[p3] = q;
             continue;
@@ -856,6 +874,21 @@ void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
         s.it = (int)a.itb[q] + itn;
         s.rr = rr;
         ss_pass<NC, double>(C, srk, s, x, y);
+        // The reference pass is the fixed-point map itself: accept the cell
+        // only if it left K where it was (max |ln K'/K| is the residual g at
+        // the polished K), so a stalled or misdirected Newton can never be
+        // written as converged.
+        double gres = 0;
+#pragma unroll
+        for (int i = 0; i < NC; i++)
+            if (z[i] != 0) gres = fmax(gres, fabs(log(s.K[i] / K[i])));
+        if (!(gres < MIXED_ACCEPT)) {
+            atomicAdd(a.counter + CNT_NT_REJECT, 1ull);
+            const unsigned long long p3 = atomicAdd(a.counter + CNT_FB_TAIL, 1ull);
+            a.list3[p3] = q;
+            continue;
+        }
+        atomicAdd(a.counter + CNT_NT_OK, 1ull);
         cell_write<NC>(a, c, s, x, y);
     }
 }

@@ -105,6 +105,19 @@ class Context:
         _check(L.osr_tune_set(self.h, key.encode(), float(value)), f"osr_tune_set({key})")
 
 
+FLASH_STATS = ("parked", "newton", "fallback", "newton_ok", "newton_boundary", "newton_nonfinite",
+               "newton_singular", "newton_diverged", "newton_maxit", "newton_rejected")
+
+
+def flash_stats(ctx, stream=None):
+    """Stage counts of the last persistent/mixed flash (dict over FLASH_STATS)."""
+    L = lib()
+    L.osr_flash_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32, C.c_void_p]
+    arr = (C.c_int64 * len(FLASH_STATS))()
+    _check(L.osr_flash_stats(ctx.h, arr, len(FLASH_STATS), _stream_ptr(stream)), "osr_flash_stats")
+    return dict(zip(FLASH_STATS, list(arr)))
+
+
 def _as_dev(t, dtype=None):
     import torch
     if not isinstance(t, torch.Tensor):

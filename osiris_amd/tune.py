@@ -1,0 +1,101 @@
+"""Launch-shape autotuner for the flash kernels.
+
+The knobs are the `osr_tune_set` keys of a context (no counterpart in the
+reference; they change scheduling only, never the arithmetic of a cell):
+
+  flash.refill_min        idle lanes a persistent wave waits for before it
+                          refills from the work queue (one atomic per refill)
+  flash.park_after        FP64 passes after which a still-iterating cell is
+                          handed to the lane-group kernel (0 = never)
+  flash.blocks_per_cu     resident blocks per CU of the persistent grid
+                          (0 = occupancy calculator)
+  flash.mixed_tol32       FP32 stage stop of the mixed path
+  flash.mixed_max32       FP32 stage pass cap of the mixed path
+
+`tune_flash` times each candidate on the caller's own cells with HIP events
+(median of a few launches) and walks the knobs one at a time, keeping the
+best value of each (coordinate descent from the built-in defaults).  The
+result is a plain dict that `apply` sets on any context of the same fluid
+and `save`/`load` keep as JSON.
+"""
+from __future__ import annotations
+
+import json
+
+from . import runtime as rt
+
+DEFAULTS = {
+    "persistent": {"flash.refill_min": 48, "flash.park_after": 40, "flash.blocks_per_cu": 0},
+    "mixed": {"flash.refill_min": 48, "flash.blocks_per_cu": 0, "flash.mixed_tol32": 1e-5,
+              "flash.mixed_max32": 200},
+}
+
+SPACE = {
+    "flash.refill_min": [8, 16, 32, 48, 64],
+    "flash.park_after": [0, 16, 24, 40, 64, 100],
+    "flash.blocks_per_cu": [0, 1, 2, 3],
+    "flash.mixed_tol32": [1e-4, 3e-5, 1e-5, 3e-6],
+    "flash.mixed_max32": [60, 120, 200, 300],
+}
+
+
+def _time(ctx, T, P, z, mode, out, reps):
+    import torch
+    stream = torch.cuda.current_stream()
+    rt.flash(ctx, T, P, z, mode=mode, out=out)          # warm-up (and workspace allocation)
+    times = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rt.flash(ctx, T, P, z, mode=mode, out=out)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    times.sort()
+    return times[len(times) // 2]
+
+
+def tune_flash(ctx, T, P, z, mode="persistent", reps=5, sweeps=1, log=None):
+    """Coordinate descent over the knobs of `mode` on cells (T, P, z) already
+    on the device.  Returns {"mode", "config", "ms", "trials"}."""
+    import torch
+    if mode not in DEFAULTS:
+        raise ValueError(f"mode must be one of {sorted(DEFAULTS)}")
+    z = rt._as_dev(z, torch.float64)
+    out = rt.alloc_flash_out(ctx.nc, z.shape[1], full=False)
+    cfg = dict(DEFAULTS[mode])
+    for k, v in cfg.items():
+        ctx.tune(k, v)
+    best = _time(ctx, T, P, z, mode, out, reps)
+    trials = [(dict(cfg), best)]
+    for _ in range(sweeps):
+        for key in cfg:
+            for v in SPACE[key]:
+                if v == cfg[key]:
+                    continue
+                ctx.tune(key, v)
+                ms = _time(ctx, T, P, z, mode, out, reps)
+                trials.append(({**cfg, key: v}, ms))
+                if log:
+                    log(f"{key}={v}: {ms:.3f} ms (best {best:.3f})")
+                if ms < best:
+                    best = ms
+                    cfg[key] = v
+            ctx.tune(key, cfg[key])
+    return {"mode": mode, "config": cfg, "ms": best, "trials": trials}
+
+
+def apply(ctx, result):
+    for k, v in result["config"].items():
+        ctx.tune(k, v)
+
+
+def save(result, path):
+    with open(path, "w") as f:
+        json.dump({"mode": result["mode"], "config": result["config"], "ms": result["ms"]}, f, indent=1)
+
+
+def load(path):
+    with open(path) as f:
+        return json.load(f)
