@@ -308,136 +308,191 @@ __device__ __forceinline__ unsigned long long wave_fetch_add(unsigned long long*
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// Lanes of one group: h = phase (0 liquid, 1 vapour), j = component slot.
+template <int S>
+struct Grp {
+    int lane, gl, gbase, h, j, jc;
+    bool valid;
+};
+
+template <int NC, int S>
+__device__ __forceinline__ Grp<S> grp_lane() {
+    Grp<S> g;
+    g.lane = (int)lane_id();
+    g.gl = g.lane % (2 * S);
+    g.gbase = g.lane - g.gl;
+    g.h = g.gl / S;
+    g.j = g.gl % S;
+    g.valid = g.j < NC;
+    g.jc = g.valid ? g.j : 0;
+    return g;
+}
+
+// max over a half-group (fmax is commutative, so every lane gets the same)
+template <int S>
+__device__ __forceinline__ double half_max(double v) {
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// (max value, smallest index among the maxima) over a half-group; v must not
+// be NaN, so every lane of the half ends with the same pair
+template <int S>
+__device__ __forceinline__ void half_argmax(double& v, int& idx) {
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+        const double ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(idx, o, 64);
+        if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+        }
+    }
+}
+
+// One reference SS pass (Init_Calcs.cpp:590-713) in group form.  Per lane:
+// z, K of component j; returns ESum = |Lb - L| (group-uniform) and leaves x,
+// y of component j (from the incoming K), Zl, Zg, L, and the updated K.
+template <int NC, int S>
+__device__ __forceinline__ double grp_pass(const FluidConsts<double>& C, bool srk, const Grp<S>& g, double z,
+                                           double sa, double bj, double pA, double pB, double& K, double& L,
+                                           double& Lb, double& x, double& y, double& Zl, double& Zg, int& rr) {
+#pragma clang fp contract(off)
+    const bool valid = g.valid;
+    // Rachford-Rice Newton from 0.6
+    double Lr = 0.6, step;
+    int jr = 0;
+    do {
+        double td = 0;
+        if (valid) td = (K >= 0) ? (1.0 - K) / (Lr + (1.0 - Lr) * K) : 1.0 / (Lr - 1.0);
+        const double Fs = half_sum<S>(z * td);
+        const double dF = -half_sum<S>(z * td * td);
+        step = Fs / dF;
+        Lr -= step;
+        jr++;
+    } while (step * step > NEWTON_TOL && jr < MAX_FLASH_N_ITR);
+    rr += jr;
+    if (Lr > 1) Lr = 1;
+    else if (Lr < 0) Lr = 0;
+    const double esum = fabs(Lb - Lr);
+    Lb = Lr;
+    L = Lr;
+    if (K >= 0) {
+        x = z / (L + K * (1.0 - L));
+        y = K * x;
+    } else {
+        x = 0;
+        y = z / (1.0 - L);
+    }
+    if (!valid) x = y = 0;
+    const double cph = g.h == 0 ? x : y;
+    const double u = sa * cph;
+    double sacc = 0;
+#pragma unroll
+    for (int k2 = 0; k2 < NC; k2++) sacc += C.bic[g.jc * NC + k2] * __shfl(u, g.gbase + g.h * S + k2, 64);
+    const double t = valid ? sa * sacc : 0.0;
+    const double Ca = half_sum<S>(u * sacc);
+    const double Cb = half_sum<S>(cph * (valid ? bj : 0.0));
+    PhaseInv<double> ph;
+    ph.A = Ca * pA;
+    ph.B = Cb * pB;
+    ph.Z = phase_z<double>(srk, ph.A, ph.B, g.h == 1);
+    ph.invCa = 1.0 / Ca;
+    ph.invCb = 1.0 / Cb;
+    ph.lnZB = log(ph.Z - ph.B);
+    const double DD = ph.A / (ph.B * (C.d1 - C.d2));
+    ph.DDFF = DD * log((ph.Z + C.d2 * ph.B) / (ph.Z + C.d1 * ph.B));
+    const double lp = lnphi<double>(ph, bj, t);
+    const double lo = __shfl_xor(lp, S, 64);
+    const double Zo = __shfl_xor(ph.Z, S, 64);
+    Zl = g.h == 0 ? ph.Z : Zo;
+    Zg = g.h == 0 ? Zo : ph.Z;
+    const double av = g.h == 0 ? lp : lo;   // ln phi liquid of component j
+    const double gv = g.h == 0 ? lo : lp;   // ln phi vapour of component j
+    if (valid && z != 0) {
+        double r;
+        if (K > 0 && x != 0 && isfinite(x)) {
+            r = exp(av - gv);
+        } else {
+            const double fL = x * exp(av);
+            const double fG = y * exp(gv);
+            r = (fG != 0) ? K * (fL / fG) : -1.0;
+        }
+        if (av != av) r = -1;
+        if (gv != gv) r = 1;
+        K = r;
+    }
+    return esum;
+}
+
+// Outputs of a cell finished in group form (Init_Calcs.cpp:717-736).
+template <int NC, int S>
+__device__ __forceinline__ void grp_write(const FlashArgs& a, const Grp<S>& g, int64_t c, double P, double T,
+                                          double x, double y, double K, double L, double Zl, double Zg, int it,
+                                          int rr) {
+    const osr_flash_out& o = a.out;
+    const int64_t n = a.n;
+    if (g.h == 0 && g.valid) {
+        if (o.x) o.x[(int64_t)g.jc * n + c] = x;
+        if (o.y) o.y[(int64_t)g.jc * n + c] = y;
+        if (o.K) o.K[(int64_t)g.jc * n + c] = K;
+    }
+    if (g.gl == 0) {
+        int8_t lab = 0;
+        if (fabs(L) < DZERO) lab = -1;
+        else if (fabs(1 - L) < DZERO) lab = 1;
+        if (o.L) o.L[c] = L;
+        if (o.label) o.label[c] = lab;
+        if (o.Zl) o.Zl[c] = Zl;
+        if (o.Zg) o.Zg[c] = Zg;
+        const double newD = Zl * L / (Zl * L + Zg * (1 - L));
+        if (o.so) o.so[c] = newD * (1 - a.sw);
+        if (o.sg) o.sg[c] = (1 - newD) * (1 - a.sw);
+        if (o.rho_l) o.rho_l[c] = P / (Zl * RGAS * T);
+        if (o.rho_g) o.rho_g[c] = P / (Zg * RGAS * T);
+        if (o.iters) o.iters[c] = it;
+        if (o.rr_iters) o.rr_iters[c] = rr;
+    }
+}
+
 template <int NC, int S>
 __global__ __launch_bounds__(256) void k_flash_group(const FluidDev* __restrict__ F, FlashArgs a) {
-#pragma clang fp contract(off)
     constexpr int G = 2 * S;
+    constexpr int GPW = 64 / G;    // groups per wave
     const FluidConsts<double>& C = F->d;
     const bool srk = F->eos == OSR_EOS_SRK;
-    const int lane = (int)lane_id();
-    const int gl = lane % G;
-    const int gbase = lane - gl;
-    const int h = gl / S;          // 0 liquid, 1 vapour
-    const int j = gl % S;          // component slot
-    const bool valid = j < NC;
-    const int jc = valid ? j : 0;
-    constexpr int GPW = 64 / G;    // groups per wave
+    const Grp<S> g = grp_lane<NC, S>();
     const unsigned long long qlen = a.counter[a.glen_slot];
     const int64_t cap = a.park_cap;
-    const double bj = C.b[jc];
+    const double bj = C.b[g.jc];
     while (true) {
         const unsigned long long base = wave_fetch_add(a.counter + CNT_GROUP, (unsigned long long)GPW);
         if (base >= qlen) break;   // wave-uniform
-        const unsigned long long qi = base + (unsigned long long)(lane / G);
+        const unsigned long long qi = base + (unsigned long long)(g.lane / G);
         if (qi >= qlen) continue;  // group-uniform: this group has no cell this round
         const int64_t q = a.glist ? a.glist[qi] : (int64_t)qi;
         const int64_t c = (int64_t)a.park[(int64_t)(NC + 4) * cap + q];
         const double T = a.T ? a.T[c] : a.T0;
         const double P = a.P[c];
-        const double z = valid ? a.z[(int64_t)jc * a.n + c] : 0.0;
-        double K = valid ? a.park[(int64_t)jc * cap + q] : 0.0;
+        const double z = g.valid ? a.z[(int64_t)g.jc * a.n + c] : 0.0;
+        double K = g.valid ? a.park[(int64_t)g.jc * cap + q] : 0.0;
         double L = a.park[(int64_t)(NC + 0) * cap + q];
         double Lb = a.park[(int64_t)(NC + 1) * cap + q];
         int it = (int)a.park[(int64_t)(NC + 2) * cap + q];
         int rr = (int)a.park[(int64_t)(NC + 3) * cap + q];
-        const double sa = valid ? C.sac[jc] * (1.0 + C.m[jc] * (1.0 - sqrt(T / C.Tc[jc]))) : 0.0;
+        const double sa = g.valid ? C.sac[g.jc] * (1.0 + C.m[g.jc] * (1.0 - sqrt(T / C.Tc[g.jc]))) : 0.0;
         const double rt = RGAS * T;
         const double pB = P / rt, pA = pB / rt;
         double x = 0, y = 0, Zl = 0, Zg = 0;
         double esum;
         do {
-            // Rachford-Rice Newton from 0.6
-            double Lr = 0.6, step;
-            int jr = 0;
-            do {
-                double td = 0;
-                if (valid) td = (K >= 0) ? (1.0 - K) / (Lr + (1.0 - Lr) * K) : 1.0 / (Lr - 1.0);
-                const double Fs = half_sum<S>(z * td);
-                const double dF = -half_sum<S>(z * td * td);
-                step = Fs / dF;
-                Lr -= step;
-                jr++;
-            } while (step * step > NEWTON_TOL && jr < MAX_FLASH_N_ITR);
-            rr += jr;
-            if (Lr > 1) Lr = 1;
-            else if (Lr < 0) Lr = 0;
-            esum = fabs(Lb - Lr);
-            Lb = Lr;
-            L = Lr;
-            if (K >= 0) {
-                x = z / (L + K * (1.0 - L));
-                y = K * x;
-            } else {
-                x = 0;
-                y = z / (1.0 - L);
-            }
-            if (!valid) x = y = 0;
-            const double cph = h == 0 ? x : y;
-            const double u = sa * cph;
-            double sacc = 0;
-#pragma unroll
-            for (int k2 = 0; k2 < NC; k2++) sacc += C.bic[jc * NC + k2] * __shfl(u, gbase + h * S + k2, 64);
-            const double t = valid ? sa * sacc : 0.0;
-            const double Ca = half_sum<S>(u * sacc);
-            const double Cb = half_sum<S>(cph * (valid ? bj : 0.0));
-            PhaseInv<double> ph;
-            ph.A = Ca * pA;
-            ph.B = Cb * pB;
-            ph.Z = phase_z<double>(srk, ph.A, ph.B, h == 1);
-            ph.invCa = 1.0 / Ca;
-            ph.invCb = 1.0 / Cb;
-            ph.lnZB = log(ph.Z - ph.B);
-            const double DD = ph.A / (ph.B * (C.d1 - C.d2));
-            ph.DDFF = DD * log((ph.Z + C.d2 * ph.B) / (ph.Z + C.d1 * ph.B));
-            const double lp = lnphi<double>(ph, bj, t);
-            const double lo = __shfl_xor(lp, S, 64);
-            const double Zo = __shfl_xor(ph.Z, S, 64);
-            Zl = h == 0 ? ph.Z : Zo;
-            Zg = h == 0 ? Zo : ph.Z;
-            const double av = h == 0 ? lp : lo;   // ln phi liquid of component j
-            const double gv = h == 0 ? lo : lp;   // ln phi vapour of component j
-            if (valid && z != 0) {
-                double r;
-                if (K > 0 && x != 0 && isfinite(x)) {
-                    r = exp(av - gv);
-                } else {
-                    const double fL = x * exp(av);
-                    const double fG = y * exp(gv);
-                    r = (fG != 0) ? K * (fL / fG) : -1.0;
-                }
-                if (av != av) r = -1;
-                if (gv != gv) r = 1;
-                K = r;
-            }
+            esum = grp_pass<NC, S>(C, srk, g, z, sa, bj, pA, pB, K, L, Lb, x, y, Zl, Zg, rr);
             it++;
         } while (esum > FLASH_TOL && it < MAX_SS_ITER);
-        // outputs (Init_Calcs.cpp:717-736)
-        const osr_flash_out& o = a.out;
-        const int64_t n = a.n;
-        if (h == 0 && valid) {
-            if (o.x) o.x[(int64_t)jc * n + c] = x;
-            if (o.y) o.y[(int64_t)jc * n + c] = y;
-            if (o.K) o.K[(int64_t)jc * n + c] = K;
-        }
-        if (gl == 0) {
-            int8_t lab = 0;
-            if (fabs(L) < DZERO) lab = -1;
-            else if (fabs(1 - L) < DZERO) lab = 1;
-            if (o.L) o.L[c] = L;
-            if (o.label) o.label[c] = lab;
-            if (o.Zl) o.Zl[c] = Zl;
-            if (o.Zg) o.Zg[c] = Zg;
-            const double newD = Zl * L / (Zl * L + Zg * (1 - L));
-            if (o.so) o.so[c] = newD * (1 - a.sw);
-            if (o.sg) o.sg[c] = (1 - newD) * (1 - a.sw);
-            if (o.rho_l) o.rho_l[c] = P / (Zl * RGAS * T);
-            if (o.rho_g) o.rho_g[c] = P / (Zg * RGAS * T);
-            if (o.iters) o.iters[c] = it;
-            if (o.rr_iters) o.rr_iters[c] = rr;
-        }
+        grp_write<NC, S>(a, g, c, P, T, x, y, K, L, Zl, Zg, it, rr);
     }
 }
-
 
 // ------------------------------------------------------- mixed precision
 // Cells still iterating after the warm FP64 passes (parked above) run:
@@ -539,6 +594,10 @@ __global__ __launch_bounds__(256) void k_mixed_ss32(const FluidDev* __restrict__
             const float L = s.L;
             if (!ok || !(L > 1e-3f && L < 1.f - 1e-3f)) to_fallback = true;
             else if (dk < a.tol32) to_newton = true;
+            // a cell not contracting to FP32 resolution within the cap may
+            // still be drifting (towards a phase boundary, say): only the
+            // reference trajectory decides where it ends, so FP64 SS resumes
+            // from its park state
             else if (npass >= a.max32) to_fallback = true;
         }
         const int64_t p2 = wave_push(a.counter + CNT_NT_TAIL, to_newton, lane, lt_mask);
@@ -558,338 +617,233 @@ __global__ __launch_bounds__(256) void k_mixed_ss32(const FluidDev* __restrict__
     }
 }
 
-// Per-phase pieces of d ln phi_i / d x_k for the cubic EOS as written in the
-// reference (ln phi_i = (b_i/Cb)(Z-1) - ln(Z-B) + q Lam (2 t_i/Ca - b_i/Cb),
-// q = A/(B(d1-d2)), Lam = ln((Z+d2 B)/(Z+d1 B)), A = Ca pA, B = Cb pB):
-//   Phi_ik = (b_i/Cb) U_k - V_k + E_i W_k - c1 t_i t_k + c2 a_ik
-// with E_i = 2 t_i/Ca - b_i/Cb, a_ik = sa_i sa_k bic_ik, c1 = 4 q Lam/Ca^2,
-// c2 = 2 q Lam/Ca, and per column k (Z_k from the implicit cubic):
-//   U_k = Z_k - b_k (Z-1)/Cb + q Lam b_k/Cb,  V_k = (Z_k - b_k pB)/(Z - B),
-//   W_k = Lam q_k + q Lam_k.
-template <int NC>
-struct PhaseDeriv {
-    float U[NC], V[NC], W[NC], t[NC];
-    float c1, c2, iCb, iCa;
-};
-
-template <int NC>
-__device__ __forceinline__ void phase_lnphi_deriv(const FluidConsts<double>& C, bool srk, const double (&sa)[NC],
-                                                  const double (&x)[NC], double pA, double pB, bool gas,
-                                                  double (&lnp)[NC], PhaseDeriv<NC>& D) {
-    double t[NC];
-    const PhaseInv<double> p = phase_eval<NC, double>(C, srk, sa, x, pA, pB, gas, t);
-#pragma unroll
-    for (int i = 0; i < NC; i++) lnp[i] = lnphi<double>(p, C.b[i], t[i]);
-    const double Z = p.Z, A = p.A, B = p.B;
-    const double u = srk ? 1.0 : 2.0, w = srk ? 0.0 : 1.0;
-    const double c2 = -1.0 + (u - 1.0) * B;
-    const double c1 = A - u * B * B - u * B - w * B * B;
-    const double FZ = 3.0 * Z * Z + 2.0 * c2 * Z + c1;
-    const double FA = Z - B;
-    const double FB = (u - 1.0) * Z * Z - (2.0 * (u + w) * B + u) * Z - A + 3.0 * w * B * B + 2.0 * w * B;
-    const double q = A / (B * (C.d1 - C.d2));
-    const double zd1 = Z + C.d1 * B, zd2 = Z + C.d2 * B;
-    const double Lam = log(zd2 / zd1);
-    const double iCa = p.invCa, iCb = p.invCb;
-    const double iFZ = 1.0 / FZ, iZB = 1.0 / (Z - B), iA = 1.0 / A, iB = 1.0 / B;
-    const double qL = q * Lam;
-#pragma unroll
-    for (int k = 0; k < NC; k++) {
-        const double Ak = 2.0 * t[k] * pA;
-        const double Bk = C.b[k] * pB;
-        const double Zk = -(FA * Ak + FB * Bk) * iFZ;
-        const double qk = q * (Ak * iA - Bk * iB);
-        const double Lk = (Zk + C.d2 * Bk) / zd2 - (Zk + C.d1 * Bk) / zd1;
-        D.U[k] = (float)(Zk - C.b[k] * (Z - 1.0) * iCb + qL * C.b[k] * iCb);
-        D.V[k] = (float)((Zk - Bk) * iZB);
-        D.W[k] = (float)(Lam * qk + q * Lk);
-        D.t[k] = (float)t[k];
-    }
-    D.c1 = (float)(4.0 * qL * iCa * iCa);
-    D.c2 = (float)(2.0 * qL * iCa);
-    D.iCb = (float)iCb;
-    D.iCa = (float)iCa;
-}
-
-// LDS row layout of the Newton system, per lane: columns [0, NC) J, NC the
-// right-hand side -g, NC+1 the rank-1 coefficient r_i, NC+2 the phase's t_i.
-template <int NC> struct NewtonLds { static constexpr int W = NC + 3; };
-
-// J[i][k] += sgn * Phi_ik * dd_k and r_i += sgn * sum_k Phi_ik uu_k.  The
-// column factors are folded into register vectors first (U_k dd_k, ...); the
-// row loop is not unrolled and reads its per-row values from LDS and from
-// scalar constant loads, so only O(NC) values are live at a time.
-template <int NC>
-__device__ __forceinline__ void add_phase_block(const FluidConsts<float>& Cf, const PhaseDeriv<NC>& D,
-                                                float Tf, const float (&dd)[NC], const float (&uu)[NC],
-                                                float sgn, float* sJ, int lane) {
-    constexpr int LW = NewtonLds<NC>::W;
-    float Uk[NC], Vk[NC], Wk[NC], Tk[NC], Sk[NC], Su[NC];
-    float Uu = 0, Vu = 0, Wu = 0, Tu = 0;
-#pragma unroll
-    for (int k = 0; k < NC; k++) {
-        Uu += D.U[k] * uu[k];
-        Vu += D.V[k] * uu[k];
-        Wu += D.W[k] * uu[k];
-        Tu += D.t[k] * uu[k];
-        const float f = sgn * dd[k];
-        Uk[k] = D.U[k] * f;
-        Vk[k] = D.V[k] * f;
-        Wk[k] = D.W[k] * f;
-        Tk[k] = D.t[k] * f;
-        const float sak = Cf.sac[k] * (1.f + Cf.m[k] * (1.f - sqrtf(Tf / Cf.Tc[k])));
-        Sk[k] = sak * f;
-        Su[k] = sak * uu[k];
-        sJ[(k * LW + NC + 2) * 64 + lane] = D.t[k];
-    }
-#pragma unroll 1
-    for (int i = 0; i < NC; i++) {
-        const float bi = Cf.b[i] * D.iCb;
-        const float ti = sJ[(i * LW + NC + 2) * 64 + lane];
-        const float Ei = 2.f * ti * D.iCa - bi;
-        const float sai = Cf.sac[i] * (1.f + Cf.m[i] * (1.f - sqrtf(Tf / Cf.Tc[i])));
-        const float c2i = D.c2 * sai;
-        const float c1i = D.c1 * ti;
-        float bicu = 0;
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-            const float bik = Cf.bic[i * NC + k];
-            sJ[(i * LW + k) * 64 + lane] += bi * Uk[k] - Vk[k] + Ei * Wk[k] - c1i * Tk[k] + c2i * bik * Sk[k];
-            bicu += bik * Su[k];
-        }
-        sJ[(i * LW + NC + 1) * 64 + lane] += sgn * (bi * Uu - Vu + Ei * Wu - c1i * Tu + c2i * bicu);
-    }
-}
-
+// Stage C in group form (2S lanes per cell, lane (h, j) = phase h,
+// component j).  The unknowns are ln K; the residual is
+//   g_i = ln K_i - ln phiL_i(x) + ln phiG_i(y)
+// with x, y from the reference's RR (Newton from 0.6, |step| <= 1e-4), whose
+// L(K) is differentiated forward through its iterations (lane j carries
+// dL/d lnK_j).  The Jacobian
+//   J = I - PhiL (dX + u_x w^T) + PhiG (dY + u_y w^T)
+// (Phi = d ln phi / d x of the cubic EOS, dX = d x / d lnK at fixed L,
+// u_x = d x / dL, w = dL / d lnK) is assembled row-wise: lane (h, i) forms
+// phase h's part of row i from per-column factors gathered by shuffle, the
+// two halves add, and the rows are factorised with partial pivoting across
+// the lanes of a half (both halves hold identical rows and take identical
+// decisions, so control flow stays group-uniform).  All FP64.  Phi of one
+// phase, with E_i = 2 t_i/Ca - b_i/Cb, q = A/(B(d1-d2)), Lam = ln((Z+d2 B)/
+// (Z+d1 B)) and Z_k from the implicit cubic:
+//   Phi_ik = (b_i/Cb) U_k - V_k + E_i W_k - c1 t_i t_k + c2 a_ik,
+//   U_k = Z_k - b_k (Z-1)/Cb + q Lam b_k/Cb, V_k = (Z_k - b_k pB)/(Z - B),
+//   W_k = Lam q_k + q Lam_k, c1 = 4 q Lam/Ca^2, c2 = 2 q Lam/Ca.
 constexpr int MIXED_NEWTON_MAX = 8;
 constexpr double MIXED_ACCEPT = 1e-10;   // max |ln K'/K| of the final reference pass
 
-// Stage C.  One lane per cell, per-lane work loop (no cross-lane operation);
-// J [NC][NC+1] (augmented with -g) lives in LDS as FP32, column-interleaved
-// by lane so the 64 lanes of the (single-wave) block never share a bank row.
-template <int NC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
-    constexpr int LW = NewtonLds<NC>::W;
-    __shared__ float sJ[NC * LW * 64];
+template <int NC, int S>
+__global__ __launch_bounds__(256) void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
+#pragma clang fp contract(off)
+    constexpr int G = 2 * S;
+    constexpr int GPW = 64 / G;
     const FluidConsts<double>& C = F->d;
-    const FluidConsts<float>& Cf = F->f;
     const bool srk = F->eos == OSR_EOS_SRK;
-    const int lane = threadIdx.x;
-    const int64_t len = (int64_t)a.counter[CNT_NT_TAIL];
+    const Grp<S> g = grp_lane<NC, S>();
+    const int h = g.h, j = g.j;
+    const bool valid = g.valid;
+    const int hb = g.gbase + h * S;    // first lane of this lane's half
+    const unsigned long long qlen = a.counter[CNT_NT_TAIL];
     const int64_t cap = a.park_cap;
-#define JM(r, c) sJ[((r) * LW + (c)) * 64 + lane]
-    // per-lane work loop with no cross-lane operation (each lane's atomic is
-    // its own; the compiler merges them into one per wave)
+    const double bj = C.b[g.jc];
+    const double uE = srk ? 1.0 : 2.0, wE = srk ? 0.0 : 1.0;   // U = uE B, W^2 = wE B^2
     while (true) {
-        const int64_t item = (int64_t)atomicAdd(a.counter + CNT_NT_HEAD, 1ull);
-        if (item >= len) break;
-        const int64_t q = a.list2[item];
+        const unsigned long long base = wave_fetch_add(a.counter + CNT_NT_HEAD, (unsigned long long)GPW);
+        if (base >= qlen) break;   // wave-uniform
+        const unsigned long long qi = base + (unsigned long long)(g.lane / G);
+        if (qi >= qlen) continue;  // group-uniform
+        const int64_t q = a.list2[qi];
         const int64_t c = (int64_t)a.park[(int64_t)(NC + 4) * cap + q];
         const double T = a.T ? a.T[c] : a.T0;
         const double P = a.P[c];
-        double z[NC], K[NC], sa[NC];
-#pragma unroll
-        for (int i = 0; i < NC; i++) {
-            z[i] = a.z[(int64_t)i * a.n + c];
-            K[i] = a.kb[(int64_t)i * cap + q];
-        }
-        sqrt_a<NC, double>(C, T, sa);
+        const double z = valid ? a.z[(int64_t)g.jc * a.n + c] : 0.0;
+        double K = valid ? a.kb[(int64_t)g.jc * cap + q] : 1.0;
+        const double sa = valid ? C.sac[g.jc] * (1.0 + C.m[g.jc] * (1.0 - sqrt(T / C.Tc[g.jc]))) : 0.0;
         const double rt = RGAS * T;
         const double pB = P / rt, pA = pB / rt;
+        const bool active = valid && z != 0;     // rows of absent components stay identity
         int rr = (int)a.itb[cap + q];
-        bool ok = false;
         int why = CNT_NT_MAXIT;
+        bool ok = false;
         int itn = 0;
         for (; itn < MIXED_NEWTON_MAX; itn++) {
-            // Rachford-Rice exactly as the reference pass, with dL/dlnK carried along
-            double L = 0.6, step;
-            double dL[NC];
-#pragma unroll
-            for (int m = 0; m < NC; m++) dL[m] = 0;
-            int j = 0;
+            // --- RR exactly as the reference pass, dL/dlnK_j carried by lane j
+            double L = 0.6, step, dL = 0;
+            int jr = 0;
             do {
-                double Fs = 0, dF = 0, sB = 0;
-#pragma unroll
-                for (int i = 0; i < NC; i++) {
-                    const double td = (1.0 - K[i]) / (L + (1.0 - L) * K[i]);
-                    Fs += z[i] * td;
-                    dF -= z[i] * td * td;
-                    sB -= z[i] * td * td * td;
+                double td = 0, gK = 0;
+                if (valid) {
+                    const double D = L + (1.0 - L) * K;
+                    td = (1.0 - K) / D;
+                    gK = -z * K / (D * D);             // z_j d td_j / d lnK_j
                 }
+                const double Fs = half_sum<S>(z * td);
+                const double dF = -half_sum<S>(z * td * td);
+                const double sB = -half_sum<S>(z * td * td * td);
                 step = Fs / dF;
-                const double idF2 = 1.0 / (dF * dF);
-#pragma unroll
-                for (int m = 0; m < NC; m++) {
-                    // d td_i/dL = -td_i^2, d td_m/d lnK_m = -K_m/D_m^2
-                    const double D = L + (1.0 - L) * K[m];
-                    const double td = (1.0 - K[m]) / D;
-                    const double gK = -z[m] * K[m] / (D * D);
-                    const double dFm = dF * dL[m] + gK;
-                    const double ddFm = -2.0 * (sB * dL[m] + td * gK);
-                    dL[m] -= (dFm * dF - Fs * ddFm) * idF2;
-                }
+                const double dFm = dF * dL + gK;
+                const double ddFm = -2.0 * (sB * dL + td * gK);
+                dL -= (dFm * dF - Fs * ddFm) / (dF * dF);
                 L -= step;
-                j++;
-            } while (step * step > NEWTON_TOL && j < MAX_FLASH_N_ITR);
-            rr += j;
+                jr++;
+            } while (step * step > NEWTON_TOL && jr < MAX_FLASH_N_ITR);
+            rr += jr;
             if (!(L > 1e-3 && L < 1.0 - 1e-3)) {   // near or past a boundary: FP64 SS decides
                 why = CNT_NT_BOUNDARY;
                 break;
             }
-            double x[NC], g[NC];
-            float dd[NC], uu[NC], w[NC];
+            // --- compositions and their derivatives (own component)
+            const double D = L + K * (1.0 - L);
+            const double x = valid ? z / D : 0.0;
+            const double iD2 = 1.0 / (D * D);
+            const double dxd = valid ? -z * K * (1.0 - L) * iD2 : 0.0;   // d x_j / d lnK_j (L fixed)
+            const double ux = valid ? -z * (1.0 - K) * iD2 : 0.0;        // d x_j / dL
+            const double cph = h == 0 ? x : K * x;
+            const double dd = h == 0 ? dxd : K * (dxd + x);
+            const double uu = h == 0 ? ux : K * ux;
+            // --- phase h: mixing, Z, ln phi_j, column factors of d ln phi / dx
+            const double u = sa * cph;
+            double sacc = 0;
 #pragma unroll
-            for (int i = 0; i < NC; i++) {
-                const double D = L + K[i] * (1.0 - L);
-                x[i] = z[i] / D;
-                const double iD2 = 1.0 / (D * D);
-                dd[i] = (float)(-z[i] * K[i] * (1.0 - L) * iD2);   // d x_i / d lnK_i (L fixed)
-                uu[i] = (float)(-z[i] * (1.0 - K[i]) * iD2);       // d x_i / d L
-                w[i] = (float)dL[i];
+            for (int k2 = 0; k2 < NC; k2++) sacc += C.bic[g.jc * NC + k2] * __shfl(u, hb + k2, 64);
+            const double t = valid ? sa * sacc : 0.0;
+            const double Ca = half_sum<S>(u * sacc);
+            const double Cb = half_sum<S>(cph * (valid ? bj : 0.0));
+            const double A = Ca * pA, B = Cb * pB;
+            const double Z = phase_z<double>(srk, A, B, h == 1);
+            const double iCa = 1.0 / Ca, iCb = 1.0 / Cb;
+            const double q = A / (B * (C.d1 - C.d2));
+            const double zd1 = Z + C.d1 * B, zd2 = Z + C.d2 * B;
+            const double Lam = log(zd2 / zd1);
+            const double lp = (bj * iCb) * (Z - 1.0) - log(Z - B) + q * Lam * (2.0 * t * iCa - bj * iCb);
+            const double c2p = -1.0 + (uE - 1.0) * B;
+            const double c1p = A - uE * B * B - uE * B - wE * B * B;
+            const double FZ = 3.0 * Z * Z + 2.0 * c2p * Z + c1p;
+            const double FA = Z - B;
+            const double FB = (uE - 1.0) * Z * Z - (2.0 * (uE + wE) * B + uE) * Z - A + 3.0 * wE * B * B + 2.0 * wE * B;
+            const double qL = q * Lam;
+            double Uc = 0, Vc = 0, Wc = 0;
+            if (valid) {
+                const double Ak = 2.0 * t * pA, Bk = bj * pB;
+                const double Zk = -(FA * Ak + FB * Bk) / FZ;
+                const double qk = q * (Ak / A - Bk / B);
+                const double Lk = (Zk + C.d2 * Bk) / zd2 - (Zk + C.d1 * Bk) / zd1;
+                Uc = Zk - bj * (Z - 1.0) * iCb + qL * bj * iCb;
+                Vc = (Zk - Bk) / (Z - B);
+                Wc = Lam * qk + q * Lk;
             }
+            const double c1 = 4.0 * qL * iCa * iCa, c2 = 2.0 * qL * iCa;
+            const double Uu = half_sum<S>(Uc * uu), Vu = half_sum<S>(Vc * uu);
+            const double Wu = half_sum<S>(Wc * uu), Tu = half_sum<S>(t * uu);
+            // --- phase h's part of row i = j
+            const double sgn = h == 0 ? -1.0 : 1.0;
+            const double bi = bj * iCb;
+            const double Ei = 2.0 * t * iCa - bi;
+            const double c1i = c1 * t, c2i = c2 * sa;
+            double row[NC];
+            double bicu = 0;
 #pragma unroll
-            for (int i = 0; i < NC; i++)
-#pragma unroll
-                for (int k = 0; k <= NC + 1; k++) JM(i, k) = (i == k) ? 1.f : 0.f;
-            const float Tf = (float)T;
-            {
-                PhaseDeriv<NC> D;
-                phase_lnphi_deriv<NC>(C, srk, sa, x, pA, pB, false, g, D);
-                add_phase_block<NC>(Cf, D, Tf, dd, uu, -1.f, sJ, lane);
+            for (int k2 = 0; k2 < NC; k2++) {
+                const int src = hb + k2;
+                const double Ud = __shfl(Uc * dd, src, 64), Vd = __shfl(Vc * dd, src, 64);
+                const double Wd = __shfl(Wc * dd, src, 64), Td = __shfl(t * dd, src, 64);
+                const double Sd = __shfl(sa * dd, src, 64), Su = __shfl(sa * uu, src, 64);
+                const double bik = C.bic[g.jc * NC + k2];
+                row[k2] = sgn * (bi * Ud - Vd + Ei * Wd - c1i * Td + c2i * bik * Sd);
+                bicu += bik * Su;
             }
+            double rpart = sgn * (bi * Uu - Vu + Ei * Wu - c1i * Tu + c2i * bicu);
+            // --- combine the halves: J row i and the rank-1 coefficient
+            const double lo = __shfl_xor(lp, S, 64);
+            const double lnL = h == 0 ? lp : lo, lnG = h == 0 ? lo : lp;
+            rpart += __shfl_xor(rpart, S, 64);
+            const double wj = (double)dL;
 #pragma unroll
-            for (int i = 0; i < NC; i++) {
-                g[i] = log(K[i]) - g[i];                         // ln K_i - ln phiL_i
-                dd[i] = (float)K[i] * (dd[i] + (float)x[i]);     // d y_i / d lnK_i
-                uu[i] = (float)K[i] * uu[i];                     // d y_i / d L
-                x[i] = K[i] * x[i];                              // y
+            for (int k2 = 0; k2 < NC; k2++) {
+                const double other = __shfl_xor(row[k2], S, 64);
+                const double wk = __shfl(wj, g.gbase + k2, 64);
+                const double v = (h == 0 ? row[k2] + other : other + row[k2]) + rpart * wk;
+                row[k2] = (k2 == j ? 1.0 : 0.0) + v;
+                if (!active) row[k2] = (k2 == j) ? 1.0 : 0.0;
             }
-            {
-                double lg[NC];
-                PhaseDeriv<NC> D;
-                phase_lnphi_deriv<NC>(C, srk, sa, x, pA, pB, true, lg, D);
-                add_phase_block<NC>(Cf, D, Tf, dd, uu, 1.f, sJ, lane);
-#pragma unroll
-                for (int i = 0; i < NC; i++) g[i] += lg[i];
-            }
-            bool fin = true;
-#pragma unroll
-            for (int i = 0; i < NC; i++) {
-                const double gi = (z[i] != 0) ? g[i] : 0.0;
-                fin = fin && isfinite(gi);
-                const float ri = JM(i, NC + 1);
-#pragma unroll
-                for (int k = 0; k < NC; k++) {
-                    if (z[i] == 0) JM(i, k) = (i == k) ? 1.f : 0.f;
-                    else JM(i, k) += ri * w[k];
-                }
-                JM(i, NC) = (float)(-gi);
-            }
-            if (!fin) {
+            const double gi = active ? log(K) - lnL + lnG : 0.0;
+            double rhs = -gi;
+            const double bad = half_max<S>(isfinite(gi) ? 0.0 : 1.0);
+            if (bad != 0.0) {
                 why = CNT_NT_NONFINITE;
                 break;
             }
-            // LU with partial pivoting on the augmented [J | -g]
+            // --- LU with partial pivoting across the lanes of the half
             bool sing = false;
 #pragma unroll
-            for (int k = 0; k < NC; k++) {
-                int piv = k;
-                float best = fabsf(JM(k, k));
+            for (int k2 = 0; k2 < NC; k2++) {
+                double cand = (valid && j >= k2) ? fabs(row[k2]) : -1.0;
+                if (cand != cand) cand = -1.0;
+                int piv = j;
+                half_argmax<S>(cand, piv);
+                if (!(cand > 0.0)) sing = true;
+                // swap rows k2 and piv (lane k2 takes piv's row and vice versa)
+                const int src = (j == k2) ? piv : (j == piv ? k2 : j);
 #pragma unroll
-                for (int rw = k + 1; rw < NC; rw++) {
-                    const float v = fabsf(JM(rw, k));
-                    if (v > best) { best = v; piv = rw; }
-                }
-                if (!(best > 0.f)) sing = true;
-                if (piv != k) {
+                for (int cc = k2; cc < NC; cc++) row[cc] = __shfl(row[cc], hb + src, 64);
+                rhs = __shfl(rhs, hb + src, 64);
+                // eliminate below the pivot row
+                const double pkk = __shfl(row[k2], hb + k2, 64);
+                const double prh = __shfl(rhs, hb + k2, 64);
+                const double f = (valid && j > k2) ? row[k2] / pkk : 0.0;
 #pragma unroll
-                    for (int cc = k; cc <= NC; cc++) {
-                        const float t0 = JM(k, cc);
-                        JM(k, cc) = JM(piv, cc);
-                        JM(piv, cc) = t0;
-                    }
-                }
-                const float ip = 1.f / JM(k, k);
-#pragma unroll
-                for (int rw = k + 1; rw < NC; rw++) {
-                    const float f = JM(rw, k) * ip;
-#pragma unroll
-                    for (int cc = k + 1; cc <= NC; cc++) JM(rw, cc) -= f * JM(k, cc);
-                }
+                for (int cc = k2 + 1; cc < NC; cc++) row[cc] -= f * __shfl(row[cc], hb + k2, 64);
+                rhs -= f * prh;
             }
             if (sing) {
                 why = CNT_NT_SINGULAR;
                 break;
             }
-            float dlt[NC];
+            // back substitution: lane k2 owns unknown k2
+            double acc = rhs, delta = 0;
 #pragma unroll
-            for (int k = NC - 1; k >= 0; k--) {
-                float acc = JM(k, NC);
-#pragma unroll
-                for (int cc = k + 1; cc < NC; cc++) acc -= JM(k, cc) * dlt[cc];
-                dlt[k] = acc / JM(k, k);
+            for (int k2 = NC - 1; k2 >= 0; k2--) {
+                const double dk = __shfl(acc / row[k2], hb + k2, 64);
+                if (j == k2) delta = dk;
+                if (j < k2) acc -= row[k2] * dk;
             }
-            double dmax = 0;
-#pragma unroll
-            for (int i = 0; i < NC; i++) {
-                if (z[i] != 0) {
-                    K[i] *= exp((double)dlt[i]);
-                    dmax = fmax(dmax, fabs((double)dlt[i]));
-                }
-            }
+            if (!active) delta = 0;
+            const double dmax = half_max<S>(fabs(delta));
             if (!isfinite(dmax) || dmax > 1.0) {
                 why = CNT_NT_DIVERGED;
                 break;
             }
+            if (active) K *= exp(delta);
             if (dmax < 1e-9) {
                 ok = true;
                 itn++;
                 break;
             }
         }
-#undef JM
-        if (!ok) {
+        if (ok) {
+            // final reference pass from the polished K; accept only a fixed point
+            const double K0 = K;
+            double L = 0, Lb = 10, x = 0, y = 0, Zl = 0, Zg = 0;
+            grp_pass<NC, S>(C, srk, g, z, sa, bj, pA, pB, K, L, Lb, x, y, Zl, Zg, rr);
+            const double gres = half_max<S>(active ? fabs(log(K / K0)) : 0.0);
+            if (gres < MIXED_ACCEPT) {
+                grp_write<NC, S>(a, g, c, P, T, x, y, K, L, Zl, Zg, (int)a.itb[q] + itn, rr);
+                if (g.gl == 0) atomicAdd(a.counter + CNT_NT_OK, 1ull);
+                continue;
+            }
+            why = CNT_NT_REJECT;
+        }
+        if (g.gl == 0) {
             atomicAdd(a.counter + why, 1ull);
             const unsigned long long p3 = atomicAdd(a.counter + CNT_FB_TAIL, 1ull);
             a.list3[p3] = q;
-            continue;
         }
-        // final reference pass from the polished K
-        SSCell<NC, double> s;
-        double x[NC], y[NC];
-#pragma unroll
-        for (int i = 0; i < NC; i++) {
-            s.K[i] = K[i];
-            s.z[i] = z[i];
-            s.sa[i] = sa[i];
-        }
-        s.T = T;
-        s.P = P;
-        s.pA = pA;
-        s.pB = pB;
-        s.L = 0;
-        s.Lb = 10;
-        s.Zl = s.Zg = 0;
-        s.it = (int)a.itb[q] + itn;
-        s.rr = rr;
-        ss_pass<NC, double>(C, srk, s, x, y);
-        // The reference pass is the fixed-point map itself: accept the cell
-        // only if it left K where it was (max |ln K'/K| is the residual g at
-        // the polished K), so a stalled or misdirected Newton can never be
-        // written as converged.
-        double gres = 0;
-#pragma unroll
-        for (int i = 0; i < NC; i++)
-            if (z[i] != 0) gres = fmax(gres, fabs(log(s.K[i] / K[i])));
-        if (!(gres < MIXED_ACCEPT)) {
-            atomicAdd(a.counter + CNT_NT_REJECT, 1ull);
-            const unsigned long long p3 = atomicAdd(a.counter + CNT_FB_TAIL, 1ull);
-            a.list3[p3] = q;
-            continue;
-        }
-        atomicAdd(a.counter + CNT_NT_OK, 1ull);
-        cell_write<NC>(a, c, s, x, y);
     }
 }
 
@@ -960,14 +914,13 @@ static int launch_nc(osr_ctx* ctx, const FlashArgs& a, int mode, hipStream_t str
         if (gB > need) gB = need;
         if (gB < 1) gB = 1;
         hipLaunchKernelGGL(k_mixed_ss32<NC>, dim3((unsigned)gB), dim3(block), 0, stream, ctx->d_fluid, b);
-        int pn = 0;
-        OSR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pn, k_mixed_newton<NC>, 64, 0));
-        if (pn < 1) pn = 1;
-        int64_t gN = (int64_t)pn * ctx->num_cu;
-        const int64_t needN = (a.n + 63) / 64;
-        if (gN > needN) gN = needN;
+        constexpr int SN = NC <= 8 ? 8 : 16;
+        const int64_t gpbN = 256 / (2 * SN);
+        int64_t gN = (int64_t)ctx->num_cu * 8;
+        const int64_t maxN = (a.n + gpbN - 1) / gpbN;
+        if (gN > maxN) gN = maxN;
         if (gN < 1) gN = 1;
-        hipLaunchKernelGGL(k_mixed_newton<NC>, dim3((unsigned)gN), dim3(64), 0, stream, ctx->d_fluid, b);
+        hipLaunchKernelGGL((k_mixed_newton<NC, SN>), dim3((unsigned)gN), dim3(256), 0, stream, ctx->d_fluid, b);
     }
     if (park_after > 0) {
         constexpr int S = NC <= 8 ? 8 : 16;

@@ -9,6 +9,7 @@ reference; they change scheduling only, never the arithmetic of a cell):
                           handed to the lane-group kernel (0 = never)
   flash.blocks_per_cu     resident blocks per CU of the persistent grid
                           (0 = occupancy calculator)
+  flash.mixed_warm_iters  FP64 passes of the mixed path before its FP32 stage
   flash.mixed_tol32       FP32 stage stop of the mixed path
   flash.mixed_max32       FP32 stage pass cap of the mixed path
 
@@ -26,8 +27,9 @@ from . import runtime as rt
 
 DEFAULTS = {
     "persistent": {"flash.refill_min": 48, "flash.park_after": 40, "flash.blocks_per_cu": 0},
-    "mixed": {"flash.refill_min": 48, "flash.blocks_per_cu": 0, "flash.mixed_tol32": 1e-5,
-              "flash.mixed_max32": 200},
+    "mixed": {"flash.refill_min": 48, "flash.blocks_per_cu": 0, "flash.mixed_tol32": 1e-4,
+              "flash.mixed_warm_iters": 3,
+              "flash.mixed_max32": 150},
 }
 
 SPACE = {
@@ -35,7 +37,8 @@ SPACE = {
     "flash.park_after": [0, 16, 24, 40, 64, 100],
     "flash.blocks_per_cu": [0, 1, 2, 3],
     "flash.mixed_tol32": [1e-4, 3e-5, 1e-5, 3e-6],
-    "flash.mixed_max32": [60, 120, 200, 300],
+    "flash.mixed_max32": [40, 80, 150, 250],
+    "flash.mixed_warm_iters": [2, 3, 5, 8],
 }
 
 
@@ -99,3 +102,36 @@ def save(result, path):
 def load(path):
     with open(path) as f:
         return json.load(f)
+
+
+def main(argv=None):
+    """Tune both launch forms on the batched-flash workload (8 pseudo-
+    components, PR, p within +/-15 % of Psat) and print the results."""
+    import argparse
+    import torch
+    from . import synth
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nc", type=int, default=8)
+    ap.add_argument("--eos", default="PR")
+    ap.add_argument("--cells", type=int, default=2_000_000)
+    ap.add_argument("--out", default=None, help="write the results as JSON here")
+    args = ap.parse_args(argv)
+    fl = synth.fluid_pseudo(args.nc, getattr(synth, args.eos), 1)
+    ctx = rt.Context(fl)
+    cells = synth.flash_cells(fl, args.cells, 7, rt.psat_numpy(ctx))
+    T = torch.tensor(cells.T, device="cuda")
+    P = torch.tensor(cells.P, device="cuda")
+    z = torch.tensor(cells.z, device="cuda")
+    res = {}
+    for mode in ("persistent", "mixed"):
+        r = tune_flash(ctx, T, P, z, mode=mode, log=lambda m: print(mode, m, flush=True))
+        res[mode] = {"config": r["config"], "ms": r["ms"]}
+        print(mode, "best", r["config"], f"{r['ms']:.3f} ms", flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(res, f, indent=1)
+    return res
+
+
+if __name__ == "__main__":
+    main()
