@@ -7,7 +7,10 @@ at +/-15 % of each cell's saturation pressure, so convergence is uneven
 (single-phase cells stop after 2 passes, two-phase ones after tens to 500).
 A step is one flash of every cell (SuccFalsh semantics, FP64) with inputs
 already resident in HBM.  `value` is cells/s of the persistent work-queue
-kernel; the plain one-lane-per-cell FP64 form is timed beside it.
+form (FP64 passes, slow cells finished by the lane-group kernel); the plain
+one-lane-per-cell FP64 form and the mixed FP32/FP64-Newton form are timed
+beside it, and the assembly leg times residual + Jacobian per Newton
+iteration on the 64x64x8 dual-porosity grid.
 
 Prints ONE JSON line (rank 0).  Only one GPU is used; --gpus > 1 exits.
 """
@@ -222,7 +225,7 @@ def main():
         "roofline": {
             "bound": "mfma",
             "bound_note": "FP64 vector ALU; MI355X FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s",
-            "kernel": f"k_flash_persistent<{NC}>",
+            "kernel": f"k_flash_persistent<{NC}> + k_flash_group<{NC}, 8> (one flash step)",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
