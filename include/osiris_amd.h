@@ -234,6 +234,26 @@ int osr_props_device(osr_asm *a, const osr_state_dev *s, double *out, void *stre
  * "flash.mixed_max32" (FP32 stage pass cap). */
 int osr_tune_set(osr_ctx *ctx, const char *key, double value);
 
+/* Tangent of the flash at a converged state (no counterpart in the
+ * reference; the sensitivities of the phase split SuccFalsh computes,
+ * Init_Calcs.cpp:571-738).  By the implicit-function theorem on
+ * g(ln K; z, P) = ln K - ln phiL(x) + ln phiG(y) = 0 with the exact
+ * Rachford-Rice split.  Inputs: T (or T0), P, z [nc][n] and the state the
+ * flash returned, K [nc][n] and label [n]; a perturbation dP [n], dz [nc][n]
+ * (NULL = 0).  Outputs dx, dy [nc][n], dL [n] (NULL = not written).
+ * Single-phase cells: the phase composition is z, L is constant, the absent
+ * phase's composition moves with K held fixed. */
+int osr_flash_tangent_device(osr_ctx *ctx, int64_t n, const double *T, double T0, const double *P,
+                             const double *z, const double *K, const int8_t *label, const double *dP,
+                             const double *dz, double *dx, double *dy, double *dL, void *stream);
+
+/* Adjoint (vector-Jacobian product) of the same map: cotangents xbar, ybar
+ * [nc][n], Lbar [n] (NULL = 0) -> Pbar [n], zbar [nc][n]. */
+int osr_flash_adjoint_device(osr_ctx *ctx, int64_t n, const double *T, double T0, const double *P,
+                             const double *z, const double *K, const int8_t *label, const double *xbar,
+                             const double *ybar, const double *Lbar, double *Pbar, double *zbar,
+                             void *stream);
+
 /* Stage counts of the last persistent/mixed flash on this context (no
  * counterpart in the reference), out[0..nout): [0] cells handed on after the
  * warm FP64 passes, [1] cells that reached the FP64 Newton polish, [2] cells

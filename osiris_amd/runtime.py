@@ -168,6 +168,70 @@ def flash(ctx, T, P, z, sw=0.0, mode="persistent", out=None, stream=None):
     return out
 
 
+def _bind_ad(L):
+    if getattr(L, "_ad_bound", False):
+        return
+    L.osr_flash_tangent_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_double] + [C.c_void_p] * 10
+    L.osr_flash_adjoint_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_double] + [C.c_void_p] * 10
+    L._ad_bound = True
+
+
+def _temp(T, n):
+    import torch
+    if np.isscalar(T):
+        return None, float(T)
+    Tt = _as_dev(T, torch.float64)
+    if Tt.shape != (n,):
+        raise ValueError("T must be scalar or [n]")
+    return Tt, 0.0
+
+
+def flash_tangent(ctx, T, P, z, K, label, dP=None, dz=None, stream=None):
+    """Tangent (dx, dy, dL) of the flash at the converged state (K, label)
+    for a perturbation (dP [n], dz [nc, n]); osr_flash_tangent_device."""
+    import torch
+    L = lib()
+    _bind_ad(L)
+    z = _as_dev(z, torch.float64)
+    nc, n = z.shape
+    P = _as_dev(P, torch.float64)
+    K = _as_dev(K, torch.float64)
+    label = _as_dev(label, torch.int8)
+    Tt, T0 = _temp(T, n)
+    dP = None if dP is None else _as_dev(dP, torch.float64)
+    dz = None if dz is None else _as_dev(dz, torch.float64)
+    dx = torch.empty(nc, n, dtype=torch.float64, device="cuda")
+    dy = torch.empty_like(dx)
+    dL = torch.empty(n, dtype=torch.float64, device="cuda")
+    _check(L.osr_flash_tangent_device(ctx.h, n, _ptr(Tt), T0, _ptr(P), _ptr(z), _ptr(K), _ptr(label),
+                                      _ptr(dP), _ptr(dz), _ptr(dx), _ptr(dy), _ptr(dL), _stream_ptr(stream)),
+           "osr_flash_tangent_device")
+    return dx, dy, dL
+
+
+def flash_adjoint(ctx, T, P, z, K, label, xbar=None, ybar=None, Lbar=None, stream=None):
+    """Adjoint (Pbar, zbar) of the flash at the converged state for the
+    cotangents of x, y [nc, n] and L [n]; osr_flash_adjoint_device."""
+    import torch
+    L = lib()
+    _bind_ad(L)
+    z = _as_dev(z, torch.float64)
+    nc, n = z.shape
+    P = _as_dev(P, torch.float64)
+    K = _as_dev(K, torch.float64)
+    label = _as_dev(label, torch.int8)
+    Tt, T0 = _temp(T, n)
+    xbar = None if xbar is None else _as_dev(xbar, torch.float64)
+    ybar = None if ybar is None else _as_dev(ybar, torch.float64)
+    Lbar = None if Lbar is None else _as_dev(Lbar, torch.float64)
+    Pbar = torch.empty(n, dtype=torch.float64, device="cuda")
+    zbar = torch.empty(nc, n, dtype=torch.float64, device="cuda")
+    _check(L.osr_flash_adjoint_device(ctx.h, n, _ptr(Tt), T0, _ptr(P), _ptr(z), _ptr(K), _ptr(label),
+                                      _ptr(xbar), _ptr(ybar), _ptr(Lbar), _ptr(Pbar), _ptr(zbar),
+                                      _stream_ptr(stream)), "osr_flash_adjoint_device")
+    return Pbar, zbar
+
+
 def psat(ctx, T, P, kind, x, y, stream=None):
     """Saturation pressure (kind 1 bubble of x, -1 dew of y).  Returns (psat, xm)."""
     import torch
