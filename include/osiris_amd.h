@@ -218,6 +218,17 @@ int osr_unpack_device(osr_asm *a, const osr_state_mut_dev *s, const double *X, v
 /* InitGuess2 (PetscAUX.cpp:878-964): state -> unknown vector X [nrows]. */
 int osr_pack_device(osr_asm *a, const osr_state_dev *s, double *X, void *stream);
 
+/* Adjoint of the residual with respect to log-permeability (no counterpart
+ * in the reference): g_perm_m[d][c] = sum_r Fbar_r dF_r / d ln perm_m[d][c],
+ * likewise g_perm_f, for a cotangent Fbar [nrows] in the row layout of the
+ * last osr_asm_layout (e.g. F itself on the fracture balance rows and 0
+ * elsewhere gives the gradient of 1/2 |F_frac,mass|^2).  Permeability enters
+ * F through the face kA/h of IFK_Calc (Init_Calcs.cpp:21-42) and the
+ * transfer shape factor (ShapeFactor_2, :970-983); state held fixed.
+ * Either output may be NULL. */
+int osr_residual_perm_vjp_device(osr_asm *a, const osr_state_dev *s, const double *Fbar, double *g_perm_m,
+                                 double *g_perm_f, void *stream);
+
 /* Cell properties after the sweep, [16+2nc][2n] (device): porpor, Pw, Po, Pg,
  * IFT, rho_o, rho_g, mu_o, mu_g, gamma_o, gamma_g, krw, kro, krg, Pcw, Pcg,
  * D_o[nc], D_g[nc]. */

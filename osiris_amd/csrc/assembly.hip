@@ -479,7 +479,7 @@ template <int NC, class TS, class TO>
 __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G, const FaceGeo& FG, int q, int c,
                                            int o, const VarsT<NC, TS>& vs, const PropsT<NC, TS>& ps,
                                            const VarsT<NC, TO>& vo, const PropsT<NC, TO>& po,
-                                           RowAcc<NC, decltype(TS() + TO())>& R) {
+                                           RowAcc<NC, decltype(TS() + TO())>& R, bool fick = true) {
     using T = decltype(TS() + TO());
     const AsmConsts& K = *A.K;
     const int d = q >> 1;
@@ -540,6 +540,7 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
             R.bal[i] -= cu * base;
         }
     }
+    if (!fick) return;
     // Fickian inter-cell flux of the components (tortuosity 1 in the fracture)
     const double dl = FG.dl, dn = FG.dn;
     const double NN = 2.0 * FG.area / (dn + dl);
@@ -573,7 +574,7 @@ template <int NC, class TM, class TF>
 __device__ __forceinline__ void transfer_piece(const AsmArgs& A, int k, const Geo& G, int c,
                                                const VarsT<NC, TM>& vm, const PropsT<NC, TM>& pm,
                                                const VarsT<NC, TF>& vf, const PropsT<NC, TF>& pf,
-                                               RowAcc<NC, decltype(TM() + TF())>& R) {
+                                               RowAcc<NC, decltype(TM() + TF())>& R, bool diffuse = true) {
     using T = decltype(TM() + TF());
     const AsmConsts& K = *A.K;
     const int n = A.n;
@@ -598,6 +599,11 @@ __device__ __forceinline__ void transfer_piece(const AsmArgs& A, int k, const Ge
     R.bal[NC] -= sgn * convw;
     const int lm = vm.lab, lf = vf.lab;
     const double tor = A.tor[c];
+    if (!diffuse) {
+#pragma unroll
+        for (int i = 0; i < NC - 1; i++) R.bal[i] -= (vm.x[i] * convo + vm.y[i] * convg) * sgn;
+        return;
+    }
     const T mo0 = (sf1 * Vb / tor) * pm.porpor * vm.S[1] * pm.ro[0];
     const T mg0 = (sf1 * Vb / tor) * pm.porpor * vm.S[2] * pm.ro[1];
     const T fo0 = (sf3 * Vb / tor) * pf.porpor * vf.S[1] * pf.ro[0];
@@ -1066,6 +1072,127 @@ __global__ __launch_bounds__(256) void k_jac_nb(AsmArgs A, const double* dp, con
     scatter_col<NC>(A, G, k, c, blk, lab, slot, d, j, R, rowptr, cols, vals);
 }
 
+// ------------------------------------------------ permeability adjoint
+// d(Fbar . F)/d ln k.  Permeability enters F only through the face
+// transmissibilities Kf = SI_C 2 A / (d_a/k_a + d_b/k_b) (IFK_Calc,
+// Init_Calcs.cpp:21-42), which scale the convective face fluxes, and through
+// the transfer shape factor sf2 = 4 sigma sum_d k_m,d / ds_d^2, which scales
+// the convective transfer terms; both enter linearly, so d F / d Kf is the
+// convective piece divided by Kf (same for sf2).
+//   k_perm_face: one lane per (continuum, cell, direction) owning the face
+//     to the + neighbour: s = sum over both blocks' balance rows of
+//     Fbar . (convective piece)/Kf; writes s dKf/dln k of the lower cell and
+//     of the upper cell to two face slots;
+//   k_perm_gather: per cell and direction, the lower-face slot of the cell
+//     plus the upper-face slot of its - neighbour (no atomics);
+//   k_perm_transfer: per cell, sum over both continua's balance rows of
+//     Fbar . (convective transfer)/sf2 times d sf2 / d ln k_m,d.
+template <int NC>
+__device__ __forceinline__ double fbar_dot(const AsmArgs& A, const double* Fbar, int blk, int lab,
+                                           const RowAcc<NC, double>& R) {
+    const int r0 = lab == 0 ? NC : 0;
+    const double* fb = Fbar + A.boff[blk] + r0;
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i <= NC; i++) s += fb[i] * R.bal[i];
+    return s;
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_perm_face(AsmArgs A, const double* Fbar, const double* perm_m,
+                                                   const double* perm_f, double* slot_lo, double* slot_hi) {
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (item >= (int64_t)2 * 3 * n) return;
+    const int c = (int)(item % n);
+    const int d = (int)((item / n) % 3);
+    const int k = (int)(item / (3 * (int64_t)n));
+    const int64_t out = ((int64_t)k * 3 + d) * n + c;
+    slot_lo[out] = 0;
+    slot_hi[out] = 0;
+    const Geo G = geom(A, c);
+    const FaceGeo FG = face_geo(A, G, 2 * d);
+    const int o = FG.o;
+    if (o < 0) return;
+    const double Kf = A.iftran[((int64_t)k * 3 + d) * n + c];
+    if (!(Kf > 0)) return;   // no flow through the face (zero porosity or zero area)
+    VarsT<NC, double> vc, vo;
+    PropsT<NC, double> pc, po;
+    load_vars<NC>(A, k, c, vc);
+    load_vars<NC>(A, k, o, vo);
+    load_props<NC>(A, k, c, pc);
+    load_props<NC>(A, k, o, po);
+    double s = 0;
+    if (A.phi[k * n + c] >= 1e-10) {
+        RowAcc<NC, double> R;
+        face_piece<NC, double, double>(A, k, G, FG, 2 * d, c, o, vc, pc, vo, po, R, false);
+        s += fbar_dot<NC>(A, Fbar, k * n + c, vc.lab, R);
+    }
+    if (A.phi[k * n + o] >= 1e-10) {
+        const Geo Go = geom(A, o);
+        const FaceGeo FGo = face_geo(A, Go, 2 * d + 1);
+        RowAcc<NC, double> R;
+        face_piece<NC, double, double>(A, k, Go, FGo, 2 * d + 1, o, c, vo, po, vc, pc, R, false);
+        s += fbar_dot<NC>(A, Fbar, k * n + o, vo.lab, R);
+    }
+    s /= Kf;
+    // dKf/dln k_c = Kf (d_c/k_c) / (d_c/k_c + d_o/k_o), likewise for o
+    const double* pk = (k == 0 ? perm_m : perm_f) + (int64_t)d * n;
+    const double rc = FG.dl / pk[c], ro = FG.dn / pk[o];
+    slot_lo[out] = s * Kf * rc / (rc + ro);
+    slot_hi[out] = s * Kf * ro / (rc + ro);
+}
+
+__global__ void k_perm_gather(const AsmArgs A, const double* slot_lo, const double* slot_hi, double* g_m,
+                              double* g_f) {
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (item >= (int64_t)2 * 3 * n) return;
+    const int c = (int)(item % n);
+    const int d = (int)((item / n) % 3);
+    const int k = (int)(item / (3 * (int64_t)n));
+    const Geo G = geom(A, c);
+    const int lower = nb_of(A, G, 2 * d + 1);   // the - neighbour owns the face below
+    double v = slot_lo[((int64_t)k * 3 + d) * n + c];
+    if (lower >= 0) v += slot_hi[((int64_t)k * 3 + d) * n + lower];
+    double* g = k == 0 ? g_m : g_f;
+    if (g) g[(int64_t)d * n + c] = v;
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_perm_transfer(AsmArgs A, const double* Fbar, double* g_m) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (c >= n || !g_m) return;
+    const AsmConsts& K = *A.K;
+    const Geo G = geom(A, c);
+    const double ds[3] = {G.ds0, G.ds1, G.ds2};
+    const double sf2 = 4 * K.sigma * (A.perm_m[c] / (ds[0] * ds[0]) + A.perm_m[n + c] / (ds[1] * ds[1]) +
+                                      A.perm_m[2 * n + c] / (ds[2] * ds[2]));
+    VarsT<NC, double> vm, vf;
+    PropsT<NC, double> pm, pf;
+    load_vars<NC>(A, 0, c, vm);
+    load_vars<NC>(A, 1, c, vf);
+    load_props<NC>(A, 0, c, pm);
+    load_props<NC>(A, 1, c, pf);
+    double s = 0;
+    if (A.phi[c] >= 1e-10) {
+        RowAcc<NC, double> R;
+        transfer_piece<NC, double, double>(A, 0, G, c, vm, pm, vf, pf, R, false);
+        s += fbar_dot<NC>(A, Fbar, c, vm.lab, R);
+    }
+    if (A.phi[n + c] >= 1e-10) {
+        RowAcc<NC, double> R;
+        transfer_piece<NC, double, double>(A, 1, G, c, vm, pm, vf, pf, R, false);
+        s += fbar_dot<NC>(A, Fbar, n + c, vf.lab, R);
+    }
+    if (!(sf2 > 0)) return;
+    s /= sf2;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        g_m[(int64_t)d * n + c] += s * 4 * K.sigma * A.perm_m[(int64_t)d * n + c] / (ds[d] * ds[d]);
+}
+
 // block sizes of the layout
 __global__ void k_block_sizes(const int8_t* lm, const int8_t* lf, int n, int nc, int64_t* bs) {
     const int blk = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1091,6 +1218,8 @@ struct osr_asm {
     int64_t* boff = nullptr;       // [2n+1]
     int64_t* bs = nullptr;         // [2n]
     double* dprops = nullptr;      // Jacobian workspace [2n][NF][2nc+4]
+    double* perm_f = nullptr;      // [3n] fracture permeabilities (permeability adjoint)
+    double* pslots = nullptr;      // permeability adjoint face slots [2][2][3][n]
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     int64_t nrows = -1;
@@ -1197,6 +1326,24 @@ int jacobian_nc(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, int32
     const int64_t gn = (nb * 7 * MAXC + 255) / 256;
     hipLaunchKernelGGL(osr::k_jac_nb<NC>, dim3((unsigned)gn), dim3(256), 0, st, A, (const double*)a->dprops,
                        rowptr, cols, vals);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+template <int NC>
+int perm_vjp_nc(osr_asm* a, const osr_state_dev* s, const double* Fbar, double* g_m, double* g_f, hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    int rc = run_props<NC>(a, A, st);
+    if (rc) return rc;
+    const int64_t n = a->n;
+    if (!a->pslots) OSR_HIP(hipMalloc((void**)&a->pslots, (size_t)2 * 2 * 3 * n * sizeof(double)));
+    double* lo = a->pslots;
+    double* hi = a->pslots + 2 * 3 * n;
+    const int64_t g6 = (6 * n + 255) / 256;
+    hipLaunchKernelGGL(osr::k_perm_face<NC>, dim3((unsigned)g6), dim3(256), 0, st, A, Fbar, (const double*)a->perm_m,
+                       (const double*)a->perm_f, lo, hi);
+    hipLaunchKernelGGL(osr::k_perm_gather, dim3((unsigned)g6), dim3(256), 0, st, A, (const double*)lo,
+                       (const double*)hi, g_m, g_f);
+    hipLaunchKernelGGL(osr::k_perm_transfer<NC>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, Fbar, g_m);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -1323,6 +1470,7 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
               upload(&a->dy, g->dy, g->ny) == hipSuccess && upload(&a->dz, g->dz, g->nz) == hipSuccess &&
               upload(&a->h, hh.data(), hh.size()) == hipSuccess && upload(&a->phi, phi.data(), phi.size()) == hipSuccess &&
               upload(&a->perm_m, permm.data(), permm.size()) == hipSuccess &&
+              upload(&a->perm_f, g->perm_f, 3 * (size_t)n) == hipSuccess &&
               upload(&a->iftran, ift.data(), ift.size()) == hipSuccess &&
               upload(&a->tor, tor.data(), tor.size()) == hipSuccess &&
               hipMalloc((void**)&a->props, (size_t)nf * 2 * n * sizeof(double)) == hipSuccess &&
@@ -1340,7 +1488,7 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
 int osr_asm_destroy(osr_asm* a) {
     if (!a) return OSR_E_ARG;
     void* ptrs[] = {a->dK, a->dx, a->dy, a->dz, a->h, a->phi, a->perm_m, a->iftran, a->tor,
-                    a->props, a->rho_tmp, a->boff, a->bs, a->dprops, a->scan_tmp};
+                    a->props, a->rho_tmp, a->boff, a->bs, a->dprops, a->perm_f, a->pslots, a->scan_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete a;
@@ -1411,6 +1559,16 @@ int osr_jacobian_device(osr_asm* a, const osr_state_dev* s, const int64_t* rowpt
     if (rc) return rc;
     if (!rowptr || !cols || !vals) return OSR_E_ARG;
 #define CALL(NC) jacobian_nc<NC>(a, s, rowptr, cols, vals, (hipStream_t)stream)
+    OSR_NC_SWITCH(a->nc, CALL)
+#undef CALL
+}
+
+int osr_residual_perm_vjp_device(osr_asm* a, const osr_state_dev* s, const double* Fbar, double* g_perm_m,
+                                 double* g_perm_f, void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!Fbar || (!g_perm_m && !g_perm_f)) return OSR_E_ARG;
+#define CALL(NC) perm_vjp_nc<NC>(a, s, Fbar, g_perm_m, g_perm_f, (hipStream_t)stream)
     OSR_NC_SWITCH(a->nc, CALL)
 #undef CALL
 }

@@ -297,6 +297,8 @@ def _bind_asm(L):
     # osr_state_mut_dev has the field layout of osr_state_dev
     L.osr_unpack_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
     L.osr_pack_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
+    L.osr_residual_perm_vjp_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]
     L._asm_bound = True
 
 
@@ -418,6 +420,21 @@ class Assembler:
         _check(lib().osr_unpack_device(self.h, C.byref(sd), C.c_void_p(X.data_ptr()),
                                        _stream_ptr(stream)), "osr_unpack_device")
         return d
+
+    def perm_vjp(self, d, Fbar, stream=None):
+        """(g_perm_m, g_perm_f) [3, n]: d(Fbar . F)/d ln perm at fixed state."""
+        import torch
+        if self.nrows is None:
+            self.layout(d, stream)
+        if Fbar.numel() != self.nrows:
+            raise ValueError(f"Fbar has {Fbar.numel()} entries, layout has {self.nrows} rows")
+        gm = torch.empty(3, self.n, dtype=torch.float64, device="cuda")
+        gf = torch.empty_like(gm)
+        sd = self._sd(d)
+        _check(lib().osr_residual_perm_vjp_device(self.h, C.byref(sd), C.c_void_p(Fbar.data_ptr()),
+                                                  C.c_void_p(gm.data_ptr()), C.c_void_p(gf.data_ptr()),
+                                                  _stream_ptr(stream)), "osr_residual_perm_vjp_device")
+        return gm, gf
 
     def props(self, d, stream=None):
         import torch
