@@ -218,6 +218,33 @@ int osr_unpack_device(osr_asm *a, const osr_state_mut_dev *s, const double *X, v
 /* InitGuess2 (PetscAUX.cpp:878-964): state -> unknown vector X [nrows]. */
 int osr_pack_device(osr_asm *a, const osr_state_dev *s, double *X, void *stream);
 
+/* Reduced (nc+1)^2 block-CSR of the Newton system (no counterpart in the reference, whose PETSc
+ * solve takes the full AIJ matrix with point-block Jacobi, PetscAUX.cpp:
+ * 628-646).  Each block's local rows (fugacity and constraint rows, which
+ * involve its own unknowns only) are eliminated:
+ *   ds = -S dp - t,   S = J_ls^-1 J_lp,   t = J_ls^-1 F_l,
+ *   sum_d (J_mp,ad - J_ms,ad S_d) dp_d = -F_m,a + sum_d J_ms,ad t_d.
+ * Primary unknowns per block (nc+1, in this order): two-phase x_0..x_{nc-3},
+ * P, Sw, So; liquid x_0..x_{nc-2}, P, Sw; vapour y_0..y_{nc-2}, P, Sw.  Rows:
+ * the nc+1 balance rows (nc-1 components, total, water) in reference order.
+ * Zero-porosity blocks reduce to an identity block with zero right-hand side.
+ *
+ * osr_bsr_reduced_rowptr_device: browptr [2n+1] (device), *nnzb (host).
+ * osr_jacobian_bsr_reduced_device: from this state's residual F and CSR
+ * (rowptr, vals of osr_jacobian_device) -> bcols [nnzb] (block index),
+ * bvals [nnzb][(nc+1)^2] row-major, brhs [2n][nc+1], and the elimination
+ * factors S [2n][nc+3][nc+1], t [2n][nc+3] (rows beyond a block's local-row
+ * count are zero).
+ * osr_bsr_reduced_expand_device: the full Newton update du [nrows] (reference
+ * unknown layout) from the reduced solution dp [2n][nc+1]. */
+int osr_bsr_reduced_rowptr_device(osr_asm *a, const osr_state_dev *s, int64_t *browptr, int64_t *nnzb,
+                                  void *stream);
+int osr_jacobian_bsr_reduced_device(osr_asm *a, const osr_state_dev *s, const double *F, const int64_t *rowptr,
+                                    const double *vals, const int64_t *browptr, int32_t *bcols, double *bvals,
+                                    double *brhs, double *S, double *t, void *stream);
+int osr_bsr_reduced_expand_device(osr_asm *a, const osr_state_dev *s, const double *S, const double *t,
+                                  const double *dp, double *du, void *stream);
+
 /* Adjoint of the residual with respect to log-permeability (no counterpart
  * in the reference): g_perm_m[d][c] = sum_r Fbar_r dF_r / d ln perm_m[d][c],
  * likewise g_perm_f, for a cotangent Fbar [nrows] in the row layout of the

@@ -1193,6 +1193,225 @@ __global__ __launch_bounds__(256) void k_perm_transfer(AsmArgs A, const double* 
         g_m[(int64_t)d * n + c] += s * 4 * K.sigma * A.perm_m[(int64_t)d * n + c] / (ds[d] * ds[d]);
 }
 
+// ------------------------------------------- reduced (nc+1)^2 block CSR
+// The local rows of a block (fugacity and constraint rows; they involve the
+// block's own unknowns only) are eliminated: with primary unknowns p (nc+1
+// per block) and secondary unknowns s,
+//   J_ls ds + J_lp dp = -F_l  =>  ds = -S dp - t,  S = J_ls^-1 J_lp, t = J_ls^-1 F_l,
+// and the balance rows become
+//   sum_d (J_mp,ad - J_ms,ad S_d) dp_d = -F_m,a + sum_d J_ms,ad t_d.
+// Primary unknowns (in this order; the rest of the block is secondary):
+//   two-phase    x_0..x_{nc-3}, P, Sw, So   (secondary x_{nc-2}, x_{nc-1}, y, Sg)
+//   liquid       x_0..x_{nc-2}, P, Sw       (secondary x_{nc-1}, So)
+//   vapour       y_0..y_{nc-2}, P, Sw       (secondary y_{nc-1}, Sg)
+// sum S = 1 is the only local row with saturations, so exactly one
+// saturation can be secondary; the compositions carry the rest.  Zero-
+// porosity blocks (identity rows, F = 0) reduce to an identity block.
+__device__ __forceinline__ int red_prim_col(int lab, int nc, int k) {
+    if (lab == 0) return k < nc - 2 ? k : 2 * nc + (k - (nc - 2));   // x.., P, Sw, So
+    return k < nc - 1 ? k : nc + (k - (nc - 1));                     // comp.., P, Sw
+}
+__device__ __forceinline__ int red_sec_col(int lab, int nc, int i) {
+    if (lab == 0) return i < 2 + nc ? nc - 2 + i : 2 * nc + 3;       // x_{nc-2}, x_{nc-1}, y.., Sg
+    return i == 0 ? nc - 1 : nc + 2;                                 // comp_{nc-1}, So|Sg
+}
+
+// entry (row r of block blk, column col of dependency block d) of the CSR;
+// sm, sw: the dependency's segment start in the balance rows and the water
+// row (seg_start, kinds 0 and 1)
+template <int NC>
+__device__ __forceinline__ double csr_at(const AsmArgs& A, int blk, int lab, int d, int64_t sm, int64_t sw, int r,
+                                         int col, const int64_t* rowptr, const double* vals) {
+    const int r0 = lab == 0 ? NC : 0;
+    const int64_t rb = A.boff[blk];
+    if (r >= r0 && r < r0 + NC) return vals[rowptr[rb + r] + sm + col];
+    if (r == r0 + NC) {
+        if (d == blk) return vals[rowptr[rb + r] + sw + col];
+        const int n = A.n;
+        const int pcol = (A.lab[d / n][d % n] == 0) ? 2 * NC : NC;
+        if (col != pcol && col != pcol + 1) return 0.0;
+        return vals[rowptr[rb + r] + sw + (col - pcol)];
+    }
+    return d == blk ? vals[rowptr[rb + r] + col] : 0.0;   // local rows: own block only
+}
+
+// one lane per block: S, t of the local elimination (partial pivoting on a
+// workspace laid out [entry][block] so the lanes' accesses coalesce)
+template <int NC>
+__global__ __launch_bounds__(256) void k_reduce_local(AsmArgs A, const double* F, const int64_t* rowptr,
+                                                      const double* vals, double* W, double* S, double* t) {
+    constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (blk >= 2 * n) return;
+    const int64_t nb = 2 * (int64_t)n;
+    const int k = blk / n, c = blk % n;
+    const int lab = A.lab[k][c];
+    double* So = S + (int64_t)blk * NLM * NP;
+    double* to = t + (int64_t)blk * NLM;
+    for (int i = 0; i < NLM * NP; i++) So[i] = 0;
+    for (int i = 0; i < NLM; i++) to[i] = 0;
+    if (A.phi[k * n + c] < 1e-10) return;   // identity rows: S = 0, t = 0
+    const int nl = nloc(lab, NC);
+    const int64_t rb = A.boff[blk];
+#define WM(r, q) W[((int64_t)(r) * NW + (q)) * nb + blk]
+    for (int i = 0; i < nl; i++) {
+        const int row = loc_row(lab, NC, i);
+        for (int j = 0; j < nl; j++) WM(i, j) = vals[rowptr[rb + row] + red_sec_col(lab, NC, j)];
+        for (int q = 0; q < NP; q++) WM(i, nl + q) = vals[rowptr[rb + row] + red_prim_col(lab, NC, q)];
+        WM(i, nl + NP) = F[rb + row];
+    }
+    const int nr = nl + NP + 1;
+    for (int q = 0; q < nl; q++) {
+        int piv = q;
+        double best = fabs(WM(q, q));
+        for (int r = q + 1; r < nl; r++) {
+            const double v = fabs(WM(r, q));
+            if (v > best) { best = v; piv = r; }
+        }
+        if (piv != q)
+            for (int cc = q; cc < nr; cc++) {
+                const double tmp = WM(q, cc);
+                WM(q, cc) = WM(piv, cc);
+                WM(piv, cc) = tmp;
+            }
+        const double ip = 1.0 / WM(q, q);
+        for (int r = q + 1; r < nl; r++) {
+            const double f = WM(r, q) * ip;
+            if (f == 0) continue;
+            for (int cc = q + 1; cc < nr; cc++) WM(r, cc) -= f * WM(q, cc);
+        }
+    }
+    for (int q = nl - 1; q >= 0; q--) {
+        const double ip = 1.0 / WM(q, q);
+        for (int cc = nl; cc < nr; cc++) {
+            double v = WM(q, cc);
+            for (int r = q + 1; r < nl; r++) v -= WM(q, r) * WM(r, cc);
+            WM(q, cc) = v * ip;
+        }
+    }
+    for (int i = 0; i < nl; i++) {
+        for (int q = 0; q < NP; q++) So[i * NP + q] = WM(i, nl + q);
+        to[i] = WM(i, nl + NP);
+    }
+#undef WM
+}
+
+// block counts of the reduced rows (zero-porosity rows keep only their own block)
+__global__ void k_reduce_bcount(AsmArgs A, int64_t* cnt) {
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (blk >= 2 * n) return;
+    const int k = blk / n, c = blk % n;
+    if (A.phi[k * n + c] < 1e-10) {
+        cnt[blk] = 1;
+        return;
+    }
+    const Geo G = geom(A, c);
+    int m = 0;
+    for (int sl = 0; sl < 8; sl++)
+        if (dep_block(A, k, G, c, sl) >= 0) m++;
+    cnt[blk] = m;
+}
+
+// one lane per (row block, dependency slot, reduced column q): column q of
+// the reduced block, and (slot 0, q 0) also the reduced right-hand side
+template <int NC>
+__global__ __launch_bounds__(256) void k_reduce_blocks(AsmArgs A, const double* F, const int64_t* rowptr,
+                                                       const double* vals, const double* S, const double* t,
+                                                       const int64_t* browptr, int32_t* bcols, double* bvals,
+                                                       double* brhs) {
+    constexpr int NLM = NC + 3, NP = NC + 1;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (item >= (int64_t)2 * n * 8 * NP) return;
+    const int q = (int)(item % NP);
+    const int slot = (int)((item / NP) % 8);
+    const int blk = (int)(item / (8 * NP));
+    const int k = blk / n, c = blk % n;
+    const Geo G = geom(A, c);
+    const int lab = A.lab[k][c];
+    const int64_t b0 = browptr[blk];
+    if (A.phi[k * n + c] < 1e-10) {
+        if (slot != 0) return;
+        bcols[b0] = blk;
+        for (int i = 0; i < NP; i++) bvals[b0 * NP * NP + i * NP + q] = (i == q) ? 1.0 : 0.0;
+        if (q == 0)
+            for (int i = 0; i < NP; i++) brhs[(int64_t)blk * NP + i] = 0.0;
+        return;
+    }
+    const int d = dep_block(A, k, G, c, slot);
+    if (d < 0) return;
+    // rank of d among the existing dependencies in block order
+    int rank = 0;
+    for (int sl = 0; sl < 8; sl++) {
+        const int e = dep_block(A, k, G, c, sl);
+        if (e >= 0 && e < d) rank++;
+    }
+    const int64_t bz = b0 + rank;
+    const int dk = d / n, dc = d % n;
+    const int dlab = A.lab[dk][dc];
+    const int dnl = A.phi[dk * n + dc] < 1e-10 ? 0 : nloc(dlab, NC);
+    const double* Sd = S + (int64_t)d * NLM * NP;
+    const int r0 = lab == 0 ? NC : 0;
+    if (q == 0) bcols[bz] = d;
+    const int64_t sm = seg_start(A, k, G, c, 0, d, NC), sw = seg_start(A, k, G, c, 1, d, NC);
+    for (int i = 0; i < NP; i++) {
+        const int r = r0 + i;
+        double v = csr_at<NC>(A, blk, lab, d, sm, sw, r, red_prim_col(dlab, NC, q), rowptr, vals);
+        for (int sidx = 0; sidx < dnl; sidx++)
+            v -= csr_at<NC>(A, blk, lab, d, sm, sw, r, red_sec_col(dlab, NC, sidx), rowptr, vals) * Sd[sidx * NP + q];
+        bvals[bz * NP * NP + i * NP + q] = v;
+    }
+    if (slot == 0 && q == 0) {
+        const int64_t rb = A.boff[blk];
+        for (int i = 0; i < NP; i++) {
+            const int r = r0 + i;
+            double v = -F[rb + r];
+            for (int sl = 0; sl < 8; sl++) {
+                const int e = dep_block(A, k, G, c, sl);
+                if (e < 0) continue;
+                const int ek = e / n, ec = e % n;
+                if (A.phi[ek * n + ec] < 1e-10) continue;
+                const int elab = A.lab[ek][ec];
+                const int enl = nloc(elab, NC);
+                const double* te = t + (int64_t)e * NLM;
+                const int64_t em = seg_start(A, k, G, c, 0, e, NC), ew = seg_start(A, k, G, c, 1, e, NC);
+                for (int sidx = 0; sidx < enl; sidx++)
+                    v += csr_at<NC>(A, blk, lab, e, em, ew, r, red_sec_col(elab, NC, sidx), rowptr, vals) * te[sidx];
+            }
+            brhs[(int64_t)blk * NP + i] = v;
+        }
+    }
+}
+
+// full Newton update from the reduced solution: du_p = dp, du_s = -S dp - t
+template <int NC>
+__global__ void k_reduce_expand(AsmArgs A, const double* S, const double* t, const double* dp, double* du) {
+    constexpr int NLM = NC + 3, NP = NC + 1;
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (blk >= 2 * n) return;
+    const int k = blk / n, c = blk % n;
+    const int lab = A.lab[k][c];
+    const int64_t rb = A.boff[blk];
+    const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
+    if (A.phi[k * n + c] < 1e-10) {
+        for (int j = 0; j < bs; j++) du[rb + j] = 0.0;
+        return;
+    }
+    const double* p = dp + (int64_t)blk * NP;
+    for (int q = 0; q < NP; q++) du[rb + red_prim_col(lab, NC, q)] = p[q];
+    const double* Sb = S + (int64_t)blk * NLM * NP;
+    const double* tb = t + (int64_t)blk * NLM;
+    const int nl = nloc(lab, NC);
+    for (int i = 0; i < nl; i++) {
+        double v = -tb[i];
+        for (int q = 0; q < NP; q++) v -= Sb[i * NP + q] * p[q];
+        du[rb + red_sec_col(lab, NC, i)] = v;
+    }
+}
+
 // block sizes of the layout
 __global__ void k_block_sizes(const int8_t* lm, const int8_t* lf, int n, int nc, int64_t* bs) {
     const int blk = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1220,6 +1439,8 @@ struct osr_asm {
     double* dprops = nullptr;      // Jacobian workspace [2n][NF][2nc+4]
     double* perm_f = nullptr;      // [3n] fracture permeabilities (permeability adjoint)
     double* pslots = nullptr;      // permeability adjoint face slots [2][2][3][n]
+    double* rwork = nullptr;       // reduced-system local elimination workspace
+    int64_t* bcnt = nullptr;       // [2n+1] block counts of the reduced rows
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     int64_t nrows = -1;
@@ -1344,6 +1565,32 @@ int perm_vjp_nc(osr_asm* a, const osr_state_dev* s, const double* Fbar, double* 
     hipLaunchKernelGGL(osr::k_perm_gather, dim3((unsigned)g6), dim3(256), 0, st, A, (const double*)lo,
                        (const double*)hi, g_m, g_f);
     hipLaunchKernelGGL(osr::k_perm_transfer<NC>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, Fbar, g_m);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+template <int NC>
+int reduced_nc(osr_asm* a, const osr_state_dev* s, const double* F, const int64_t* rowptr, const double* vals,
+               const int64_t* browptr, int32_t* bcols, double* bvals, double* brhs, double* S, double* t,
+               hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
+    const int64_t nb = 2 * (int64_t)a->n;
+    if (!a->rwork) OSR_HIP(hipMalloc((void**)&a->rwork, (size_t)NLM * NW * nb * sizeof(double)));
+    hipLaunchKernelGGL(osr::k_reduce_local<NC>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, A, F, rowptr,
+                       vals, a->rwork, S, t);
+    const int64_t g = (nb * 8 * NP + 255) / 256;
+    hipLaunchKernelGGL(osr::k_reduce_blocks<NC>, dim3((unsigned)g), dim3(256), 0, st, A, F, rowptr, vals,
+                       (const double*)S, (const double*)t, browptr, bcols, bvals, brhs);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+template <int NC>
+int expand_nc(osr_asm* a, const osr_state_dev* s, const double* S, const double* t, const double* dp, double* du,
+              hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    const int64_t nb = 2 * (int64_t)a->n;
+    hipLaunchKernelGGL(osr::k_reduce_expand<NC>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, A, S, t, dp,
+                       du);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -1488,7 +1735,8 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
 int osr_asm_destroy(osr_asm* a) {
     if (!a) return OSR_E_ARG;
     void* ptrs[] = {a->dK, a->dx, a->dy, a->dz, a->h, a->phi, a->perm_m, a->iftran, a->tor,
-                    a->props, a->rho_tmp, a->boff, a->bs, a->dprops, a->perm_f, a->pslots, a->scan_tmp};
+                    a->props, a->rho_tmp, a->boff, a->bs, a->dprops, a->perm_f, a->pslots,
+                    a->rwork, a->bcnt, a->scan_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete a;
@@ -1569,6 +1817,46 @@ int osr_residual_perm_vjp_device(osr_asm* a, const osr_state_dev* s, const doubl
     if (rc) return rc;
     if (!Fbar || (!g_perm_m && !g_perm_f)) return OSR_E_ARG;
 #define CALL(NC) perm_vjp_nc<NC>(a, s, Fbar, g_perm_m, g_perm_f, (hipStream_t)stream)
+    OSR_NC_SWITCH(a->nc, CALL)
+#undef CALL
+}
+
+int osr_bsr_reduced_rowptr_device(osr_asm* a, const osr_state_dev* s, int64_t* browptr, int64_t* nnzb,
+                                  void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!browptr) return OSR_E_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    osr::AsmArgs A = make_args(a, s);
+    const int64_t nb = 2 * (int64_t)a->n;
+    hipLaunchKernelGGL(osr::k_reduce_bcount, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, A, browptr);
+    hipLaunchKernelGGL(osr::k_set_i64, dim3(1), dim3(1), 0, st, browptr + nb, (int64_t)0);
+    rc = exclusive_scan(a, browptr, browptr, nb + 1, st);
+    if (rc) return rc;
+    if (nnzb) {
+        OSR_HIP(hipMemcpyAsync(nnzb, browptr + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        OSR_HIP(hipStreamSynchronize(st));
+    }
+    return OSR_OK;
+}
+
+int osr_jacobian_bsr_reduced_device(osr_asm* a, const osr_state_dev* s, const double* F, const int64_t* rowptr,
+                                    const double* vals, const int64_t* browptr, int32_t* bcols, double* bvals,
+                                    double* brhs, double* S, double* t, void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!F || !rowptr || !vals || !browptr || !bcols || !bvals || !brhs || !S || !t) return OSR_E_ARG;
+#define CALL(NC) reduced_nc<NC>(a, s, F, rowptr, vals, browptr, bcols, bvals, brhs, S, t, (hipStream_t)stream)
+    OSR_NC_SWITCH(a->nc, CALL)
+#undef CALL
+}
+
+int osr_bsr_reduced_expand_device(osr_asm* a, const osr_state_dev* s, const double* S, const double* t,
+                                  const double* dp, double* du, void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!S || !t || !dp || !du) return OSR_E_ARG;
+#define CALL(NC) expand_nc<NC>(a, s, S, t, dp, du, (hipStream_t)stream)
     OSR_NC_SWITCH(a->nc, CALL)
 #undef CALL
 }

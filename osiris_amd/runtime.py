@@ -299,6 +299,10 @@ def _bind_asm(L):
     L.osr_pack_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
     L.osr_residual_perm_vjp_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]
+    L.osr_bsr_reduced_rowptr_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p,
+                                                C.POINTER(C.c_int64), C.c_void_p]
+    L.osr_jacobian_bsr_reduced_device.argtypes = [C.c_void_p, C.POINTER(_StateDev)] + [C.c_void_p] * 10
+    L.osr_bsr_reduced_expand_device.argtypes = [C.c_void_p, C.POINTER(_StateDev)] + [C.c_void_p] * 5
     L._asm_bound = True
 
 
@@ -420,6 +424,41 @@ class Assembler:
         _check(lib().osr_unpack_device(self.h, C.byref(sd), C.c_void_p(X.data_ptr()),
                                        _stream_ptr(stream)), "osr_unpack_device")
         return d
+
+    def reduced(self, d, F, rowptr, vals, stream=None):
+        """Reduced (nc+1)^2 BSR of the Newton system: (browptr, bcols, bvals
+        [nnzb, nc+1, nc+1], brhs [2n, nc+1], S, t)."""
+        import torch
+        nb = 2 * self.n
+        np1 = self.nc + 1
+        browptr = torch.empty(nb + 1, dtype=torch.int64, device="cuda")
+        nnzb = C.c_int64()
+        sd = self._sd(d)
+        _check(lib().osr_bsr_reduced_rowptr_device(self.h, C.byref(sd), C.c_void_p(browptr.data_ptr()),
+                                                   C.byref(nnzb), _stream_ptr(stream)),
+               "osr_bsr_reduced_rowptr_device")
+        bcols = torch.empty(nnzb.value, dtype=torch.int32, device="cuda")
+        bvals = torch.empty(nnzb.value, np1, np1, dtype=torch.float64, device="cuda")
+        brhs = torch.empty(nb, np1, dtype=torch.float64, device="cuda")
+        S = torch.empty(nb, self.nc + 3, np1, dtype=torch.float64, device="cuda")
+        t = torch.empty(nb, self.nc + 3, dtype=torch.float64, device="cuda")
+        _check(lib().osr_jacobian_bsr_reduced_device(
+            self.h, C.byref(sd), C.c_void_p(F.data_ptr()), C.c_void_p(rowptr.data_ptr()),
+            C.c_void_p(vals.data_ptr()), C.c_void_p(browptr.data_ptr()), C.c_void_p(bcols.data_ptr()),
+            C.c_void_p(bvals.data_ptr()), C.c_void_p(brhs.data_ptr()), C.c_void_p(S.data_ptr()),
+            C.c_void_p(t.data_ptr()), _stream_ptr(stream)), "osr_jacobian_bsr_reduced_device")
+        return browptr, bcols, bvals, brhs, S, t
+
+    def reduced_expand(self, d, S, t, dp, stream=None):
+        """Full Newton update [nrows] from the reduced solution dp [2n, nc+1]."""
+        import torch
+        du = torch.empty(self.nrows, dtype=torch.float64, device="cuda")
+        sd = self._sd(d)
+        _check(lib().osr_bsr_reduced_expand_device(self.h, C.byref(sd), C.c_void_p(S.data_ptr()),
+                                                   C.c_void_p(t.data_ptr()), C.c_void_p(dp.data_ptr()),
+                                                   C.c_void_p(du.data_ptr()), _stream_ptr(stream)),
+               "osr_bsr_reduced_expand_device")
+        return du
 
     def perm_vjp(self, d, Fbar, stream=None):
         """(g_perm_m, g_perm_f) [3, n]: d(Fbar . F)/d ln perm at fixed state."""
