@@ -103,7 +103,30 @@ def assembly_leg(rt, synth, torch, args):
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a0.record(stream)
+        for _ in range(reps):
+            fn()
+        a1.record(stream)
+        torch.cuda.synchronize()
+        return a0.elapsed_time(a1) / reps
+
+    # reduced (nc+1)^2 block CSR from this iteration's residual and CSR
+    red = {}
+    ms_red = timed(lambda: red.update(r=asm.reduced(d, F, rp, vals)), steps)
+    nnzb = int(red["r"][1].shape[0])
+    # adjoint of the residual w.r.t. log-permeability (cotangent F)
+    ms_perm = timed(lambda: asm.perm_vjp(d, F), steps)
+    # flash adjoint over the grid's cells (feed: the fracture liquid composition)
     n = grid.n
+    zf = torch.as_tensor(st.x_f, device="cuda")
+    outf = rt.flash(ctx, grid.T, d["Po_f"], zf)
+    ms_adj = timed(lambda: rt.flash_adjoint(ctx, grid.T, d["Po_f"], zf, outf["K"], outf["label"],
+                                            xbar=zf, Lbar=d["Po_f"] * 0 + 1.0), steps)
     # algorithmic HBM bytes of one residual+Jacobian: state read, F write,
     # CSR write (8 B value + 4 B column) -- properties stay on chip/L2
     nc = fl.nc
@@ -118,6 +141,9 @@ def assembly_leg(rt, synth, torch, args):
         "algorithmic_bytes": state_b + out_b,
         "achieved_GBs": (state_b + out_b) / (ms * 1e-3) / 1e9,
         "hbm_peak_GBs": PEAK_HBM_GBS,
+        "reduced_bsr": {"ms": ms_red, "nnzb": nnzb, "block": f"{nc + 1}x{nc + 1}"},
+        "perm_adjoint_ms": ms_perm,
+        "flash_adjoint": {"ms": ms_adj, "cells": n},
     }
 
 
