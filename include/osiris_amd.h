@@ -181,6 +181,26 @@ typedef struct osr_asm osr_asm;
 int osr_asm_create(osr_ctx *ctx, const osr_fluid_ext *fe, const osr_grid *grid, osr_asm **out);
 int osr_asm_destroy(osr_asm *a);
 
+/* Jacobian workspace (no counterpart in the reference, whose PETSc Mat is the
+ * only buffer).  The Jacobian and reduced-system calls need scratch of
+ * *bytes ((16+2nc)(3nc+6) + (nc+3)(2nc+5) + 8(nc+1) doubles per block,
+ * 13.7 KB at nc = 10); by default each
+ * osr_asm allocates its own on first use.  osr_asm_set_workspace hands it
+ * a caller-owned device buffer instead (256-byte aligned, at least *bytes;
+ * NULL returns to a library-owned one), so assemblers whose calls are
+ * serialised on one stream -- e.g. many grids replayed in one graph -- can
+ * share one buffer sized for the largest.  Calls on different streams must
+ * not share it. */
+int osr_asm_workspace_size(osr_asm *a, size_t *bytes);
+int osr_asm_set_workspace(osr_asm *a, void *ptr, size_t bytes);
+
+/* Device pointer to the assembler's cell-property table, [16+2nc][2n] in the
+ * field order of osr_props_device, rewritten by every residual/Jacobian call
+ * on the stream it runs on (e.g. Pg = table + 3*2n for PhaseCtrl).  Valid
+ * until osr_asm_destroy.  The table replaces the per-cell property globals
+ * the reference's sweeps fill (blockFProps and kin, Do_Cycle.cpp:99-437). */
+int osr_asm_props_table(osr_asm *a, const double **props);
+
 /* Block offsets of the layout for the state's labels (PJac).  offsets
  * (device, may be NULL) receives 2n+1 entries; *nrows (host) = MatrixSize.
  * Must be called after any label change, before residual/Jacobian. */
@@ -244,6 +264,19 @@ int osr_jacobian_bsr_reduced_device(osr_asm *a, const osr_state_dev *s, const do
                                     double *brhs, double *S, double *t, void *stream);
 int osr_bsr_reduced_expand_device(osr_asm *a, const osr_state_dev *s, const double *S, const double *t,
                                   const double *dp, double *du, void *stream);
+
+/* The same reduced system (bcols, bvals, brhs, S, t as above, browptr from
+ * osr_bsr_reduced_rowptr_device) assembled straight from the state without
+ * the full CSR: the Jacobian is linear in the seed direction, so column q of
+ * a reduced block is the forward-mode derivative of the balance rows along
+ * e_{primary q} - S_d e_{secondaries}, and the right-hand-side correction
+ * the derivative along t_d on the secondaries.  F is this state's residual
+ * (osr_residual_device).  No counterpart in the reference (see above); it is
+ * the form that fits one GPU at the largest grids, where the full AIJ matrix
+ * of FormJacobian (PetscAUX.cpp:106-173) would not. */
+int osr_jacobian_bsr_reduced_direct_device(osr_asm *a, const osr_state_dev *s, const double *F,
+                                           const int64_t *browptr, int32_t *bcols, double *bvals, double *brhs,
+                                           double *S, double *t, void *stream);
 
 /* Adjoint of the residual with respect to log-permeability (no counterpart
  * in the reference): g_perm_m[d][c] = sum_r Fbar_r dF_r / d ln perm_m[d][c],

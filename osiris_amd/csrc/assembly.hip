@@ -913,17 +913,19 @@ __global__ __launch_bounds__(256) void k_dprops(AsmArgs A, double* dp) {
     for (int f = 0; f < NF; f++) o[f * MAXC] = prop_field<NC, D1>(sp, f).d[0];
 }
 
+// properties of block blk as duals: values from the property table, the
+// derivative along direction j from a workspace laid out [block][field][ncol]
 template <int NC>
 __device__ __forceinline__ void load_props_dual(const AsmArgs& A, const double* dp, int blk, int j,
-                                                PropsT<NC, DualV<1>>& p) {
-    constexpr int NF = JacShape<NC>::NF, MAXC = JacShape<NC>::MAXC;
+                                                PropsT<NC, DualV<1>>& p, int ncol = JacShape<NC>::MAXC) {
+    constexpr int NF = JacShape<NC>::NF;
     const int64_t s = 2 * (int64_t)A.n;
     const double* q = A.props + blk;
-    const double* g = dp + (int64_t)blk * NF * MAXC + j;
+    const double* g = dp + (int64_t)blk * NF * ncol + j;
     auto F = [&](int f) {
         DualV<1> r;
         r.v = q[f * s];
-        r.d[0] = g[f * MAXC];
+        r.d[0] = g[f * ncol];
         return r;
     };
     p.porpor = F(0); p.Pw = F(1); p.Po = F(2); p.Pg = F(3); p.bift = F(4);
@@ -1235,32 +1237,14 @@ __device__ __forceinline__ double csr_at(const AsmArgs& A, int blk, int lab, int
     return d == blk ? vals[rowptr[rb + r] + col] : 0.0;   // local rows: own block only
 }
 
-// one lane per block: S, t of the local elimination (partial pivoting on a
-// workspace laid out [entry][block] so the lanes' accesses coalesce)
+// Gauss elimination with partial pivoting of one block's local system,
+// held in a workspace laid out [entry][block] so the lanes' accesses
+// coalesce: rows 0..nl-1, columns [secondary 0..nl-1 | primary 0..NP-1 | F].
+// Leaves S (nl x NP) and t (nl) of ds = -S dp - t in So, to.
 template <int NC>
-__global__ __launch_bounds__(256) void k_reduce_local(AsmArgs A, const double* F, const int64_t* rowptr,
-                                                      const double* vals, double* W, double* S, double* t) {
+__device__ __forceinline__ void local_eliminate(double* W, int64_t nb, int blk, int nl, double* So, double* to) {
     constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
-    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = A.n;
-    if (blk >= 2 * n) return;
-    const int64_t nb = 2 * (int64_t)n;
-    const int k = blk / n, c = blk % n;
-    const int lab = A.lab[k][c];
-    double* So = S + (int64_t)blk * NLM * NP;
-    double* to = t + (int64_t)blk * NLM;
-    for (int i = 0; i < NLM * NP; i++) So[i] = 0;
-    for (int i = 0; i < NLM; i++) to[i] = 0;
-    if (A.phi[k * n + c] < 1e-10) return;   // identity rows: S = 0, t = 0
-    const int nl = nloc(lab, NC);
-    const int64_t rb = A.boff[blk];
 #define WM(r, q) W[((int64_t)(r) * NW + (q)) * nb + blk]
-    for (int i = 0; i < nl; i++) {
-        const int row = loc_row(lab, NC, i);
-        for (int j = 0; j < nl; j++) WM(i, j) = vals[rowptr[rb + row] + red_sec_col(lab, NC, j)];
-        for (int q = 0; q < NP; q++) WM(i, nl + q) = vals[rowptr[rb + row] + red_prim_col(lab, NC, q)];
-        WM(i, nl + NP) = F[rb + row];
-    }
     const int nr = nl + NP + 1;
     for (int q = 0; q < nl; q++) {
         int piv = q;
@@ -1295,6 +1279,35 @@ __global__ __launch_bounds__(256) void k_reduce_local(AsmArgs A, const double* F
         to[i] = WM(i, nl + NP);
     }
 #undef WM
+}
+
+// one lane per block: S, t of the local elimination, the local rows read
+// from the full CSR
+template <int NC>
+__global__ __launch_bounds__(256) void k_reduce_local(AsmArgs A, const double* F, const int64_t* rowptr,
+                                                      const double* vals, double* W, double* S, double* t) {
+    constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (blk >= 2 * n) return;
+    const int64_t nb = 2 * (int64_t)n;
+    const int k = blk / n, c = blk % n;
+    const int lab = A.lab[k][c];
+    double* So = S + (int64_t)blk * NLM * NP;
+    double* to = t + (int64_t)blk * NLM;
+    for (int i = 0; i < NLM * NP; i++) So[i] = 0;
+    for (int i = 0; i < NLM; i++) to[i] = 0;
+    if (A.phi[k * n + c] < 1e-10) return;   // identity rows: S = 0, t = 0
+    const int nl = nloc(lab, NC);
+    const int64_t rb = A.boff[blk];
+    for (int i = 0; i < nl; i++) {
+        const int row = loc_row(lab, NC, i);
+        double* w = W + (int64_t)i * NW * nb + blk;
+        for (int j = 0; j < nl; j++) w[(int64_t)j * nb] = vals[rowptr[rb + row] + red_sec_col(lab, NC, j)];
+        for (int q = 0; q < NP; q++) w[(int64_t)(nl + q) * nb] = vals[rowptr[rb + row] + red_prim_col(lab, NC, q)];
+        w[(int64_t)(nl + NP) * nb] = F[rb + row];
+    }
+    local_eliminate<NC>(W, nb, blk, nl, So, to);
 }
 
 // block counts of the reduced rows (zero-porosity rows keep only their own block)
@@ -1412,6 +1425,301 @@ __global__ void k_reduce_expand(AsmArgs A, const double* S, const double* t, con
     }
 }
 
+// ------------------------------- reduced block CSR straight from the state
+// The same reduced system without the full CSR.  The Jacobian is linear in
+// the seed direction, so the reduced column q of dependency d is the
+// directional derivative of the balance rows along
+//   v_q = e_{primary q} - sum_s S_d[s][q] e_{secondary s}   (q < NP)
+// and the right-hand-side correction sum_d J_ms,d t_d the one along
+//   w   = sum_s t_d[s] e_{secondary s}                        (q = NP).
+//   k_red_loc    one lane per (block, unknown j): the local rows' column j
+//                (own_piece in forward mode) into the elimination workspace;
+//   k_red_elim   one lane per block: S, t;
+//   k_red_dprops one lane per (block, direction q <= NP): the property
+//                derivatives along v_q / w, combined from the per-unknown
+//                ones (the property sweep is linear in the seed);
+//   k_red_own / k_red_nb  the balance rows along direction q of the own
+//                block / of dependency slot 1..7, written to the reduced
+//                block's column q, or (q = NP) to a per-slot partial sum;
+//   k_red_rhs    one lane per block: -F_m plus the slot partials.
+
+// role of unknown column j of a block: primary index (>= 0), or -1 - the
+// secondary index (the inverse of red_prim_col / red_sec_col)
+__device__ __forceinline__ int red_role(int lab, int nc, int j) {
+    if (lab == 0) {
+        if (j < nc - 2) return j;
+        if (j < 2 * nc) return -1 - (j - (nc - 2));
+        if (j <= 2 * nc + 2) return nc - 2 + (j - 2 * nc);
+        return -1 - (nc + 2);
+    }
+    if (j < nc - 1) return j;
+    if (j == nc - 1) return -1;
+    if (j <= nc + 1) return nc - 1 + (j - nc);
+    return -2;
+}
+
+// the block's unknowns seeded along reduced direction q (see above); nl is
+// the number of secondaries in play (0 for a zero-porosity block)
+template <int NC>
+__device__ __forceinline__ void seed_dir(const VarsT<NC, double>& v, const double* Sb, const double* tb, int nl,
+                                         int q, double pcoeff, VarsT<NC, DualV<1>>& out) {
+    constexpr int NP = NC + 1;
+    out.lab = v.lab;
+    out.Po = DualV<1>(v.Po);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        out.x[i] = DualV<1>(v.x[i]);
+        out.y[i] = DualV<1>(v.y[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) out.S[j] = DualV<1>(v.S[j]);
+    int col = 0;
+    auto mark = [&](DualV<1>& t, double sc) {
+        const int role = red_role(v.lab, NC, col);
+        double dv = 0.0;
+        if (role >= 0) {
+            dv = (role == q) ? 1.0 : 0.0;
+        } else {
+            const int si = -1 - role;
+            if (si < nl) dv = q < NP ? -Sb[si * NP + q] : tb[si];
+        }
+        t.d[0] = dv * sc;
+        col++;
+    };
+    // unknown order of seed_vars (InitGuess2, PetscAUX.cpp:892-957)
+    if (v.lab != -1)
+        for (int i = 0; i < NC; i++) mark(out.x[i], 1.0);
+    if (v.lab != 1)
+        for (int i = 0; i < NC; i++) mark(out.y[i], 1.0);
+    mark(out.Po, pcoeff);
+    mark(out.S[0], 1.0);
+    if (v.lab != -1) mark(out.S[1], 1.0);
+    if (v.lab != 1) mark(out.S[2], 1.0);
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, const double* dp, const double* F, double* W) {
+    using D1 = DualV<1>;
+    constexpr int MAXC = JacShape<NC>::MAXC, NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (int64_t)2 * A.n * MAXC) return;
+    const int j = (int)(item % MAXC);
+    const int blk = (int)(item / MAXC);
+    const int n = A.n, k = blk / n, c = blk % n;
+    const int lab = A.lab[k][c];
+    const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
+    if (j >= bs || A.phi[k * n + c] < 1e-10) return;
+    const Geo G = geom(A, c);
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    VarsT<NC, D1> sv;
+    seed_vars<NC, 1>(v, j, A.pcoeff, sv);
+    PropsT<NC, D1> sp;
+    load_props_dual<NC>(A, dp, blk, j, sp);
+    RowAcc<NC, D1> R;
+    own_piece<NC, D1>(A, k, c, G, sv, sp, R);
+    const int nl = nloc(lab, NC);
+    const int role = red_role(lab, NC, j);
+    const int wc = role < 0 ? -1 - role : nl + role;
+    const int64_t nb = 2 * (int64_t)n;
+#pragma unroll
+    for (int i = 0; i < NC + 3; i++)
+        if (i < nl) W[((int64_t)i * NW + wc) * nb + blk] = R.loc[i].d[0];
+    if (j == 0) {
+        const int64_t rb = A.boff[blk];
+        for (int i = 0; i < nl; i++) W[((int64_t)i * NW + nl + NP) * nb + blk] = F[rb + loc_row(lab, NC, i)];
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_red_elim(AsmArgs A, double* W, double* S, double* t) {
+    constexpr int NLM = NC + 3, NP = NC + 1;
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (blk >= 2 * n) return;
+    const int k = blk / n, c = blk % n;
+    double* So = S + (int64_t)blk * NLM * NP;
+    double* to = t + (int64_t)blk * NLM;
+    for (int i = 0; i < NLM * NP; i++) So[i] = 0;
+    for (int i = 0; i < NLM; i++) to[i] = 0;
+    if (A.phi[k * n + c] < 1e-10) return;   // identity rows: S = 0, t = 0
+    local_eliminate<NC>(W, 2 * (int64_t)n, blk, nloc(A.lab[k][c], NC), So, to);
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* dp, const double* S, const double* t,
+                                                    double* dred) {
+    constexpr int NF = JacShape<NC>::NF, MAXC = JacShape<NC>::MAXC, NLM = NC + 3, NP = NC + 1;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (int64_t)2 * A.n * (NP + 1)) return;
+    const int q = (int)(item % (NP + 1));
+    const int blk = (int)(item / (NP + 1));
+    const int n = A.n, k = blk / n, c = blk % n;
+    const int lab = A.lab[k][c];
+    const int nl = A.phi[k * n + c] < 1e-10 ? 0 : nloc(lab, NC);
+    const double* Sb = S + (int64_t)blk * NLM * NP;
+    const double* tb = t + (int64_t)blk * NLM;
+    double wgt[NC + 3];
+    int sc[NC + 3];
+#pragma unroll
+    for (int i = 0; i < NC + 3; i++) {
+        wgt[i] = i < nl ? (q < NP ? -Sb[i * NP + q] : tb[i]) : 0.0;
+        sc[i] = i < nl ? red_sec_col(lab, NC, i) : 0;
+    }
+    const int pc = q < NP ? red_prim_col(lab, NC, q) : -1;
+    const double* g = dp + (int64_t)blk * NF * MAXC;
+    double* o = dred + (int64_t)blk * NF * (NP + 1) + q;
+    for (int f = 0; f < NF; f++) {
+        const double* gf = g + f * MAXC;
+        double v = pc >= 0 ? gf[pc] : 0.0;
+#pragma unroll
+        for (int i = 0; i < NC + 3; i++)
+            if (i < nl) v += wgt[i] * gf[sc[i]];
+        o[f * (NP + 1)] = v;
+    }
+}
+
+// rank of dependency d among the row block's existing dependencies in
+// block order (its position in the reduced block row)
+__device__ __forceinline__ int dep_rank(const AsmArgs& A, int k, const Geo& G, int c, int d) {
+    int rank = 0;
+    for (int sl = 0; sl < 8; sl++) {
+        const int e = dep_block(A, k, G, c, sl);
+        if (e >= 0 && e < d) rank++;
+    }
+    return rank;
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, const double* S, const double* t,
+                                                 const int64_t* browptr, int32_t* bcols, double* bvals,
+                                                 double* rpart) {
+    using D1 = DualV<1>;
+    constexpr int NLM = NC + 3, NP = NC + 1;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (int64_t)2 * A.n * (NP + 1)) return;
+    const int q = (int)(item % (NP + 1));
+    const int blk = (int)(item / (NP + 1));
+    const int n = A.n, k = blk / n, c = blk % n;
+    const int64_t b0 = browptr[blk];
+    if (A.phi[k * n + c] < 1e-10) {
+        // identity block (Petsc_Functions.cpp:42051-42056); its right-hand side is 0
+        if (q == NP) return;
+        if (q == 0) bcols[b0] = blk;
+        for (int i = 0; i < NP; i++) bvals[b0 * NP * NP + i * NP + q] = (i == q) ? 1.0 : 0.0;
+        return;
+    }
+    const Geo G = geom(A, c);
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    VarsT<NC, D1> sv;
+    seed_dir<NC>(v, S + (int64_t)blk * NLM * NP, t + (int64_t)blk * NLM, nloc(v.lab, NC), q, A.pcoeff, sv);
+    PropsT<NC, D1> sp;
+    load_props_dual<NC>(A, dred, blk, q, sp, NP + 1);
+    RowAcc<NC, D1> R;
+    own_piece<NC, D1>(A, k, c, G, sv, sp, R);
+    for (int f = 0; f < 6; f++) {
+        const FaceGeo FG = face_geo(A, G, f);
+        const int o = FG.o;
+        if (o < 0) continue;
+        VarsT<NC, double> vo;
+        PropsT<NC, double> po;
+        load_vars<NC>(A, k, o, vo);
+        load_props<NC>(A, k, o, po);
+        face_piece<NC, D1, double>(A, k, G, FG, f, c, o, sv, sp, vo, po, R);
+    }
+    {
+        VarsT<NC, double> vx;
+        PropsT<NC, double> px;
+        load_vars<NC>(A, 1 - k, c, vx);
+        load_props<NC>(A, 1 - k, c, px);
+        if (k == 0) transfer_piece<NC, D1, double>(A, 0, G, c, sv, sp, vx, px, R);
+        else transfer_piece<NC, double, D1>(A, 1, G, c, vx, px, sv, sp, R);
+    }
+    if (q == NP) {
+        double* o = rpart + (int64_t)blk * 8 * NP;
+#pragma unroll
+        for (int i = 0; i < NP; i++) o[i] = R.bal[i].d[0];
+        return;
+    }
+    const int64_t bz = b0 + dep_rank(A, k, G, c, blk);
+    if (q == 0) bcols[bz] = blk;
+#pragma unroll
+    for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, const double* S, const double* t,
+                                                const int64_t* browptr, int32_t* bcols, double* bvals,
+                                                double* rpart) {
+    using D1 = DualV<1>;
+    constexpr int NLM = NC + 3, NP = NC + 1;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (int64_t)2 * A.n * 7 * (NP + 1)) return;
+    const int q = (int)(item % (NP + 1));
+    const int slot = 1 + (int)((item / (NP + 1)) % 7);
+    const int blk = (int)(item / (7 * (NP + 1)));
+    const int n = A.n, k = blk / n, c = blk % n;
+    if (A.phi[k * n + c] < 1e-10) return;
+    const Geo G = geom(A, c);
+    const int d = dep_block(A, k, G, c, slot);
+    if (d < 0) return;
+    const int dk = d / n, dc = d % n;
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    PropsT<NC, double> p;
+    load_props<NC>(A, k, c, p);
+    VarsT<NC, double> vd;
+    load_vars<NC>(A, dk, dc, vd);
+    const int dnl = A.phi[dk * n + dc] < 1e-10 ? 0 : nloc(vd.lab, NC);
+    VarsT<NC, D1> sv;
+    seed_dir<NC>(vd, S + (int64_t)d * NLM * NP, t + (int64_t)d * NLM, dnl, q, A.pcoeff, sv);
+    PropsT<NC, D1> sp;
+    load_props_dual<NC>(A, dred, d, q, sp, NP + 1);
+    RowAcc<NC, D1> R;
+    if (slot == 7) {
+        if (k == 0) transfer_piece<NC, double, D1>(A, 0, G, c, v, p, sv, sp, R);
+        else transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, v, p, R);
+    } else {
+        const FaceGeo FG = face_geo(A, G, slot - 1);
+        face_piece<NC, double, D1>(A, k, G, FG, slot - 1, c, dc, v, p, sv, sp, R);
+    }
+    if (q == NP) {
+        double* o = rpart + ((int64_t)blk * 8 + slot) * NP;
+#pragma unroll
+        for (int i = 0; i < NP; i++) o[i] = R.bal[i].d[0];
+        return;
+    }
+    const int64_t bz = browptr[blk] + dep_rank(A, k, G, c, d);
+    if (q == 0) bcols[bz] = d;
+#pragma unroll
+    for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
+}
+
+template <int NC>
+__global__ void k_red_rhs(AsmArgs A, const double* F, const double* rpart, double* brhs) {
+    constexpr int NP = NC + 1;
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (blk >= 2 * n) return;
+    const int k = blk / n, c = blk % n;
+    double* o = brhs + (int64_t)blk * NP;
+    if (A.phi[k * n + c] < 1e-10) {
+        for (int i = 0; i < NP; i++) o[i] = 0.0;
+        return;
+    }
+    const Geo G = geom(A, c);
+    const int r0 = A.lab[k][c] == 0 ? NC : 0;
+    const int64_t rb = A.boff[blk];
+    const double* rp = rpart + (int64_t)blk * 8 * NP;
+    for (int i = 0; i < NP; i++) {
+        double v = -F[rb + r0 + i];
+        for (int sl = 0; sl < 8; sl++)
+            if (dep_block(A, k, G, c, sl) >= 0) v += rp[sl * NP + i];
+        o[i] = v;
+    }
+}
+
 // block sizes of the layout
 __global__ void k_block_sizes(const int8_t* lm, const int8_t* lf, int n, int nc, int64_t* bs) {
     const int blk = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1441,6 +1749,9 @@ struct osr_asm {
     double* pslots = nullptr;      // permeability adjoint face slots [2][2][3][n]
     double* rwork = nullptr;       // reduced-system local elimination workspace
     int64_t* bcnt = nullptr;       // [2n+1] block counts of the reduced rows
+    double* dred = nullptr;        // direct reduction: property derivatives along the reduced directions
+    double* rpart = nullptr;       // direct reduction: per-slot right-hand-side partials [2n][8][nc+1]
+    bool ws_external = false;      // dprops, rwork, dred, rpart live in a caller's workspace
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     int64_t nrows = -1;
@@ -1581,6 +1892,36 @@ int reduced_nc(osr_asm* a, const osr_state_dev* s, const double* F, const int64_
     const int64_t g = (nb * 8 * NP + 255) / 256;
     hipLaunchKernelGGL(osr::k_reduce_blocks<NC>, dim3((unsigned)g), dim3(256), 0, st, A, F, rowptr, vals,
                        (const double*)S, (const double*)t, browptr, bcols, bvals, brhs);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+template <int NC>
+int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const int64_t* browptr, int32_t* bcols,
+                      double* bvals, double* brhs, double* S, double* t, hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    int rc = run_props<NC>(a, A, st);
+    if (rc) return rc;
+    constexpr int NF = osr::JacShape<NC>::NF, MAXC = osr::JacShape<NC>::MAXC;
+    constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
+    const int64_t nb = 2 * (int64_t)a->n;
+    if (!a->dprops) OSR_HIP(hipMalloc((void**)&a->dprops, (size_t)nb * NF * MAXC * sizeof(double)));
+    if (!a->rwork) OSR_HIP(hipMalloc((void**)&a->rwork, (size_t)NLM * NW * nb * sizeof(double)));
+    if (!a->dred) OSR_HIP(hipMalloc((void**)&a->dred, (size_t)nb * NF * (NP + 1) * sizeof(double)));
+    if (!a->rpart) OSR_HIP(hipMalloc((void**)&a->rpart, (size_t)nb * 8 * NP * sizeof(double)));
+    const unsigned g1 = (unsigned)((nb + 255) / 256);
+    const unsigned gc = (unsigned)((nb * MAXC + 255) / 256);
+    const unsigned gq = (unsigned)((nb * (NP + 1) + 255) / 256);
+    const unsigned gn = (unsigned)((nb * 7 * (NP + 1) + 255) / 256);
+    hipLaunchKernelGGL(osr::k_dprops<NC>, dim3(gc), dim3(256), 0, st, A, a->dprops);
+    hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3(gc), dim3(256), 0, st, A, (const double*)a->dprops, F, a->rwork);
+    hipLaunchKernelGGL(osr::k_red_elim<NC>, dim3(g1), dim3(256), 0, st, A, a->rwork, S, t);
+    hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dprops,
+                       (const double*)S, (const double*)t, a->dred);
+    hipLaunchKernelGGL(osr::k_red_own<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, (const double*)S,
+                       (const double*)t, browptr, bcols, bvals, a->rpart);
+    hipLaunchKernelGGL(osr::k_red_nb<NC>, dim3(gn), dim3(256), 0, st, A, (const double*)a->dred, (const double*)S,
+                       (const double*)t, browptr, bcols, bvals, a->rpart);
+    hipLaunchKernelGGL(osr::k_red_rhs<NC>, dim3(g1), dim3(256), 0, st, A, F, (const double*)a->rpart, brhs);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -1735,11 +2076,65 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
 int osr_asm_destroy(osr_asm* a) {
     if (!a) return OSR_E_ARG;
     void* ptrs[] = {a->dK, a->dx, a->dy, a->dz, a->h, a->phi, a->perm_m, a->iftran, a->tor,
-                    a->props, a->rho_tmp, a->boff, a->bs, a->dprops, a->perm_f, a->pslots,
-                    a->rwork, a->bcnt, a->scan_tmp};
+                    a->props, a->rho_tmp, a->boff, a->bs, a->perm_f, a->pslots, a->bcnt, a->scan_tmp};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    if (!a->ws_external) {
+        void* ws[] = {a->dprops, a->rwork, a->dred, a->rpart};
+        for (void* p : ws)
+            if (p) hipFree(p);
+    }
     delete a;
+    return OSR_OK;
+}
+
+// Jacobian workspace layout: [dprops | rwork | dred | rpart], 256-byte aligned
+static size_t ws_layout(const osr_asm* a, size_t off[4]) {
+    const size_t nb = 2 * (size_t)a->n, nc = (size_t)a->nc;
+    const size_t NF = 16 + 2 * nc, MAXC = 2 * nc + 4, NLM = nc + 3, NP = nc + 1, NW = NLM + NP + 1;
+    const size_t sz[4] = {nb * NF * MAXC * 8, NLM * NW * nb * 8, nb * NF * (NP + 1) * 8, nb * 8 * NP * 8};
+    size_t at = 0;
+    for (int i = 0; i < 4; i++) {
+        off[i] = at;
+        at += (sz[i] + 255) / 256 * 256;
+    }
+    return at;
+}
+
+int osr_asm_props_table(osr_asm* a, const double** props) {
+    if (!a || !props) return OSR_E_ARG;
+    *props = a->props;
+    return OSR_OK;
+}
+
+int osr_asm_workspace_size(osr_asm* a, size_t* bytes) {
+    if (!a || !bytes) return OSR_E_ARG;
+    size_t off[4];
+    *bytes = ws_layout(a, off);
+    return OSR_OK;
+}
+
+int osr_asm_set_workspace(osr_asm* a, void* ptr, size_t bytes) {
+    if (!a) return OSR_E_ARG;
+    size_t off[4];
+    const size_t need = ws_layout(a, off);
+    if (ptr && (bytes < need || ((uintptr_t)ptr & 255))) return OSR_E_ARG;
+    if (!a->ws_external) {
+        void* ws[] = {a->dprops, a->rwork, a->dred, a->rpart};
+        for (void* p : ws)
+            if (p) OSR_HIP(hipFree(p));
+    }
+    if (ptr) {
+        char* b = (char*)ptr;
+        a->dprops = (double*)(b + off[0]);
+        a->rwork = (double*)(b + off[1]);
+        a->dred = (double*)(b + off[2]);
+        a->rpart = (double*)(b + off[3]);
+        a->ws_external = true;
+    } else {
+        a->dprops = a->rwork = a->dred = a->rpart = nullptr;
+        a->ws_external = false;
+    }
     return OSR_OK;
 }
 
@@ -1847,6 +2242,17 @@ int osr_jacobian_bsr_reduced_device(osr_asm* a, const osr_state_dev* s, const do
     if (rc) return rc;
     if (!F || !rowptr || !vals || !browptr || !bcols || !bvals || !brhs || !S || !t) return OSR_E_ARG;
 #define CALL(NC) reduced_nc<NC>(a, s, F, rowptr, vals, browptr, bcols, bvals, brhs, S, t, (hipStream_t)stream)
+    OSR_NC_SWITCH(a->nc, CALL)
+#undef CALL
+}
+
+int osr_jacobian_bsr_reduced_direct_device(osr_asm* a, const osr_state_dev* s, const double* F,
+                                           const int64_t* browptr, int32_t* bcols, double* bvals, double* brhs,
+                                           double* S, double* t, void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!F || !browptr || !bcols || !bvals || !brhs || !S || !t) return OSR_E_ARG;
+#define CALL(NC) reduced_direct_nc<NC>(a, s, F, browptr, bcols, bvals, brhs, S, t, (hipStream_t)stream)
     OSR_NC_SWITCH(a->nc, CALL)
 #undef CALL
 }

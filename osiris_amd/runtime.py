@@ -303,6 +303,9 @@ def _bind_asm(L):
                                                 C.POINTER(C.c_int64), C.c_void_p]
     L.osr_jacobian_bsr_reduced_device.argtypes = [C.c_void_p, C.POINTER(_StateDev)] + [C.c_void_p] * 10
     L.osr_bsr_reduced_expand_device.argtypes = [C.c_void_p, C.POINTER(_StateDev)] + [C.c_void_p] * 5
+    L.osr_jacobian_bsr_reduced_direct_device.argtypes = [C.c_void_p, C.POINTER(_StateDev)] + [C.c_void_p] * 8
+    L.osr_asm_workspace_size.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+    L.osr_asm_set_workspace.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     L._asm_bound = True
 
 
@@ -340,6 +343,23 @@ class Assembler:
         if getattr(self, "h", None):
             lib().osr_asm_destroy(self.h)
             self.h = None
+        self._ws = None
+
+    def workspace_bytes(self):
+        b = C.c_size_t()
+        _check(lib().osr_asm_workspace_size(self.h, C.byref(b)), "osr_asm_workspace_size")
+        return b.value
+
+    def set_workspace(self, buf):
+        """Use a caller-owned device buffer (a uint8 tensor of at least
+        workspace_bytes()) as the Jacobian workspace; None returns to a
+        library-owned one.  Assemblers sharing a buffer must run on one stream."""
+        if buf is None:
+            _check(lib().osr_asm_set_workspace(self.h, None, 0), "osr_asm_set_workspace")
+        else:
+            _check(lib().osr_asm_set_workspace(self.h, C.c_void_p(buf.data_ptr()), buf.numel() * buf.element_size()),
+                   "osr_asm_set_workspace")
+        self._ws = buf          # keep the buffer alive while in use
 
     def __del__(self):
         try:
@@ -425,9 +445,7 @@ class Assembler:
                                        _stream_ptr(stream)), "osr_unpack_device")
         return d
 
-    def reduced(self, d, F, rowptr, vals, stream=None):
-        """Reduced (nc+1)^2 BSR of the Newton system: (browptr, bcols, bvals
-        [nnzb, nc+1, nc+1], brhs [2n, nc+1], S, t)."""
+    def _reduced_out(self, d, stream):
         import torch
         nb = 2 * self.n
         np1 = self.nc + 1
@@ -442,12 +460,34 @@ class Assembler:
         brhs = torch.empty(nb, np1, dtype=torch.float64, device="cuda")
         S = torch.empty(nb, self.nc + 3, np1, dtype=torch.float64, device="cuda")
         t = torch.empty(nb, self.nc + 3, dtype=torch.float64, device="cuda")
+        return browptr, bcols, bvals, brhs, S, t
+
+    def reduced(self, d, F, rowptr, vals, stream=None):
+        """Reduced (nc+1)^2 BSR of the Newton system from the full CSR:
+        (browptr, bcols, bvals [nnzb, nc+1, nc+1], brhs [2n, nc+1], S, t)."""
+        out = self._reduced_out(d, stream)
+        browptr, bcols, bvals, brhs, S, t = out
+        sd = self._sd(d)
         _check(lib().osr_jacobian_bsr_reduced_device(
             self.h, C.byref(sd), C.c_void_p(F.data_ptr()), C.c_void_p(rowptr.data_ptr()),
             C.c_void_p(vals.data_ptr()), C.c_void_p(browptr.data_ptr()), C.c_void_p(bcols.data_ptr()),
             C.c_void_p(bvals.data_ptr()), C.c_void_p(brhs.data_ptr()), C.c_void_p(S.data_ptr()),
             C.c_void_p(t.data_ptr()), _stream_ptr(stream)), "osr_jacobian_bsr_reduced_device")
-        return browptr, bcols, bvals, brhs, S, t
+        return out
+
+    def reduced_direct(self, d, F, out=None, stream=None):
+        """The same reduced system assembled straight from the state (no full
+        CSR); out: a tuple from an earlier call to refill in place."""
+        if out is None:
+            out = self._reduced_out(d, stream)
+        browptr, bcols, bvals, brhs, S, t = out
+        sd = self._sd(d)
+        _check(lib().osr_jacobian_bsr_reduced_direct_device(
+            self.h, C.byref(sd), C.c_void_p(F.data_ptr()), C.c_void_p(browptr.data_ptr()),
+            C.c_void_p(bcols.data_ptr()), C.c_void_p(bvals.data_ptr()), C.c_void_p(brhs.data_ptr()),
+            C.c_void_p(S.data_ptr()), C.c_void_p(t.data_ptr()), _stream_ptr(stream)),
+            "osr_jacobian_bsr_reduced_direct_device")
+        return out
 
     def reduced_expand(self, d, S, t, dp, stream=None):
         """Full Newton update [nrows] from the reduced solution dp [2n, nc+1]."""

@@ -170,6 +170,8 @@ class State:
     pcoeff: float = 1.0
     pre_m: np.ndarray = None        # [nc+1, n] or None
     pre_f: np.ndarray = None
+    z_m: np.ndarray = None          # [nc, n] feed the state was flashed from (or None)
+    z_f: np.ndarray = None
 
 
 def dual_porosity_case(fluid, nx, ny, nz, seed, flash_fn, dx=10.0, dy=10.0, dz=5.0,
@@ -187,16 +189,17 @@ def dual_porosity_case(fluid, nx, ny, nz, seed, flash_fn, dx=10.0, dy=10.0, dz=5
                 np.repeat(np.exp(np.log(k_m) + k_sigma * g.normal(size=n))[None], 3, 0).copy(),
                 np.repeat(np.exp(np.log(k_f) + k_sigma * g.normal(size=n))[None], 3, 0).copy(),
                 np.ones(n), SWT_TEST_DAT.copy(), SGT_TEST_DAT.copy(), T=T, **grid_kw)
-    st = {}
+    st, feeds = {}, {}
     for cont, sw in (("m", sw_m), ("f", sw_f)):
         P = g.uniform(p_range[0], p_range[1], n)
         z = g.dirichlet(np.full(fluid.nc, alpha), n).T.copy()
         r = flash_fn(T, P, z, sw)
         st[cont] = (P, r, sw)
+        feeds[cont] = z
     def S(r, sw):
         return np.stack([np.full(n, sw), r["so"], r["sg"]])
     (Pm, rm, swm), (Pf, rf, swf) = st["m"], st["f"]
     state = State(rm["label"].astype(np.int8), rf["label"].astype(np.int8), Pm, Pf,
                   rm["x"], rm["y"], rf["x"], rf["y"], S(rm, swm), S(rf, swf),
-                  pcoeff=float(max(Pm.max(), Pf.max())))
+                  pcoeff=float(max(Pm.max(), Pf.max())), z_m=feeds["m"], z_f=feeds["f"])
     return grid, state

@@ -3,7 +3,13 @@ update must solve the full Newton system J du = -F.
 
 Bars: the normwise backward error of the expanded update,
 |J du + F|_inf / (|J|_inf |du|_inf + |F|_inf), below 1e-12, and agreement
-with a sparse direct solve of the full system to 1e-7 of max |du|.  (A
+with a sparse direct solve of the full system to 1e-7 of max |du|, for the
+reduction from the full CSR and for the one assembled straight from the
+state (no CSR).  The two reductions agree: S and t bitwise (both eliminate
+the same forward-mode local rows), blocks and right-hand sides within 1e-9
+of the largest entry of their reduced row (they differ by summation order:
+J_p - J_s S summed explicitly versus one forward-mode sweep along the
+combined direction).  (A
 componentwise bar is meaningless here: at the initial state many rows --
 water rows at zero mobility, constraint rows -- hold only rounding-level
 terms, and the row scales span ~1e7.)
@@ -25,10 +31,75 @@ def _cases(oracle):
     fl6 = synth.fluid_pseudo(6, synth.PR, 3)
     yield "pr6_6x5x3", (fl6,) + synth.dual_porosity_case(fl6, 6, 5, 3, 4, oracle_flash_fn(oracle, fl6),
                                                           p_range=(8e6, 20e6))
+    fl10 = synth.fluid_pseudo(10, synth.PR, 5)
+    yield "pr10_5x4x3", (fl10,) + synth.dual_porosity_case(fl10, 5, 4, 3, 6, oracle_flash_fn(oracle, fl10),
+                                                            p_range=(8e6, 20e6))
     fl, grid, st = c0_case(oracle, nx=4, ny=4, nz=1, seed=7)
     grid.phi_m[[0, 5]] = 0.0
     grid.phi_f[[3]] = 0.0
     yield "c0_zero_porosity", (fl, grid, st)
+
+
+def _check_identity_blocks(grid, np1, red):
+    browptr, bcols, bvals, brhs, S, t = red
+    br, bc, bv = browptr.cpu().numpy(), bcols.cpu().numpy(), bvals.cpu().numpy()
+    for blk in (0, 5, grid.n + 3):
+        assert br[blk + 1] - br[blk] == 1 and bc[br[blk]] == blk
+        assert np.array_equal(bv[br[blk]], np.eye(np1))
+        assert np.all(brhs.cpu().numpy()[blk] == 0)
+        assert np.all(S.cpu().numpy()[blk] == 0) and np.all(t.cpu().numpy()[blk] == 0)
+
+
+def _check_solve(name, asm, d, F, rp, cols, vals, red, nb, np1, nrows):
+    browptr, bcols, bvals, brhs, S, t = red
+    R = sp.bsr_matrix((bvals.cpu().numpy(), bcols.cpu().numpy().astype(np.int64), browptr.cpu().numpy()),
+                      shape=(nb * np1, nb * np1)).tocsc()
+    dp = spla.spsolve(R, brhs.cpu().numpy().reshape(-1))
+    du = asm.reduced_expand(d, S, t, torch.as_tensor(dp.reshape(nb, np1)).cuda()).cpu().numpy()
+    J = sp.csr_matrix((vals.cpu().numpy(), cols.cpu().numpy(), rp.cpu().numpy()), shape=(nrows, nrows))
+    Fh = F.cpu().numpy()
+    assert np.isfinite(Fh).all(), name
+    res = J @ du + Fh
+    eta = np.abs(res).max() / (abs(J).sum(axis=1).max() * np.abs(du).max() + np.abs(Fh).max())
+    assert eta < 1e-12, (name, eta)
+    du_full = spla.spsolve(J.tocsc(), -Fh)
+    err = np.abs(du - du_full).max() / np.abs(du_full).max()
+    assert err < 1e-7, (name, err)
+
+
+def _same_reduction(name, a, b, np1, nan_rows=False):
+    """a: from the CSR, b: direct.  nan_rows: the state has the reference's
+    NaN rows (next to zero-porosity cells); the two reductions carry that
+    NaN through different products, so only rows finite in both compare."""
+    for i in (0, 1):
+        assert torch.equal(a[i], b[i]), (name, i)
+    for i, what in ((4, "S"), (5, "t")):
+        x, y = a[i].cpu().numpy(), b[i].cpu().numpy()
+        assert np.array_equal(np.isnan(x), np.isnan(y)), (name, what)
+        assert np.array_equal(np.nan_to_num(x), np.nan_to_num(y)), (name, what)
+    bv_a, bv_b = a[2].cpu().numpy(), b[2].cpu().numpy()
+    rh_a, rh_b = a[3].cpu().numpy(), b[3].cpu().numpy()
+    if not nan_rows:
+        assert np.isfinite(bv_a).all() and np.isfinite(bv_b).all(), name
+        assert np.isfinite(rh_a).all() and np.isfinite(rh_b).all(), name
+    br = a[0].cpu().numpy()
+    nb = len(br) - 1
+    worst = 0.0
+    for blk in range(nb):
+        blocks_a = bv_a[br[blk]:br[blk + 1]]            # [m, np1, np1]
+        blocks_b = bv_b[br[blk]:br[blk + 1]]
+        fin = (np.isfinite(blocks_a).all(axis=(0, 2)) & np.isfinite(blocks_b).all(axis=(0, 2))
+               & np.isfinite(rh_a[blk]) & np.isfinite(rh_b[blk]))
+        if not fin.any():
+            continue
+        scale = np.abs(blocks_a[:, fin, :]).max(axis=(0, 2))
+        scale = np.maximum(scale, np.abs(rh_a[blk, fin]))
+        scale = np.where(scale > 0, scale, 1.0)
+        e = np.abs(blocks_a[:, fin, :] - blocks_b[:, fin, :]).max(axis=(0, 2)) / scale
+        e = np.maximum(e, np.abs(rh_a[blk, fin] - rh_b[blk, fin]) / scale)
+        worst = max(worst, float(e.max()))
+    assert worst < 1e-9, (name, worst)
+    return worst
 
 
 def test_reduced_system_solves_the_full_one(gpu, oracle):
@@ -40,30 +111,19 @@ def test_reduced_system_solves_the_full_one(gpu, oracle):
         _, nrows = asm.layout(d)
         F = asm.residual(d)
         rp, cols, vals = asm.jacobian(d)
-        browptr, bcols, bvals, brhs, S, t = asm.reduced(d, F, rp, vals)
         np1 = fl.nc + 1
         nb = 2 * grid.n
+        red = asm.reduced(d, F, rp, vals)
+        red_direct = asm.reduced_direct(d, F)
+        torch.cuda.synchronize()
+        print(name, "direct vs CSR reduction, worst row-scaled difference",
+              _same_reduction(name, red, red_direct, np1, nan_rows=name == "c0_zero_porosity"), flush=True)
         if name == "c0_zero_porosity":
             # the reference's 0/0 next to zero-porosity cells puts NaN into
             # neighbouring rows, so no solve is meaningful: check the
             # zero-porosity blocks reduce to identity blocks with zero rhs
-            br, bc, bv = browptr.cpu().numpy(), bcols.cpu().numpy(), bvals.cpu().numpy()
-            for blk in (0, 5, grid.n + 3):
-                assert br[blk + 1] - br[blk] == 1 and bc[br[blk]] == blk
-                assert np.array_equal(bv[br[blk]], np.eye(np1))
-                assert np.all(brhs.cpu().numpy()[blk] == 0)
-                assert np.all(S.cpu().numpy()[blk] == 0) and np.all(t.cpu().numpy()[blk] == 0)
+            _check_identity_blocks(grid, np1, red)
+            _check_identity_blocks(grid, np1, red_direct)
             continue
-        R = sp.bsr_matrix((bvals.cpu().numpy(), bcols.cpu().numpy().astype(np.int64), browptr.cpu().numpy()),
-                          shape=(nb * np1, nb * np1)).tocsc()
-        dp = spla.spsolve(R, brhs.cpu().numpy().reshape(-1))
-        du = asm.reduced_expand(d, S, t, torch.as_tensor(dp.reshape(nb, np1)).cuda()).cpu().numpy()
-        J = sp.csr_matrix((vals.cpu().numpy(), cols.cpu().numpy(), rp.cpu().numpy()), shape=(nrows, nrows))
-        Fh = F.cpu().numpy()
-        assert np.isfinite(Fh).all(), name
-        res = J @ du + Fh
-        eta = np.abs(res).max() / (abs(J).sum(axis=1).max() * np.abs(du).max() + np.abs(Fh).max())
-        assert eta < 1e-12, (name, eta)
-        du_full = spla.spsolve(J.tocsc(), -Fh)
-        err = np.abs(du - du_full).max() / np.abs(du_full).max()
-        assert err < 1e-7, (name, err)
+        _check_solve(name, asm, d, F, rp, cols, vals, red, nb, np1, nrows)
+        _check_solve(name + " direct", asm, d, F, rp, cols, vals, red_direct, nb, np1, nrows)
