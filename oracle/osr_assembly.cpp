@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "dual.hpp"
@@ -164,7 +165,7 @@ struct Props {
 };
 
 template <class T>
-T solve_z_t(T a1, T a2, T a3, bool gas) {
+T solve_z_val(T a1, T a2, T a3, bool gas) {
     using std::acos; using std::cos; using std::pow; using std::sqrt;
     using orad::acos; using orad::cos; using orad::pow; using orad::sqrt;
     const T Q = (3 * a2 - a1 * a1) / 9;
@@ -203,6 +204,26 @@ T solve_z_t(T a1, T a2, T a3, bool gas) {
         Z = Z - a1 / 3;
     }
     return Z;
+}
+
+template <class T>
+T solve_z_t(T a1, T a2, T a3, bool gas) {
+    if constexpr (!std::is_same<T, double>::value) {
+        // Forward mode: the root's value from the branches below (the
+        // reference's arithmetic), its derivative from the implicit function
+        // theorem on f(Z) = Z^3 + a1 Z^2 + a2 Z + a3 = 0, as the reference's
+        // hand Jacobian takes dZ (Do_Cycle.cpp:438-616).  Differentiating the
+        // explicit formulas instead fails where they cancel: in the one-root
+        // branch with Q -> 0, J - sqrt(D) rounds to exactly 0 and its cube
+        // root has an infinite slope.
+        const double Z = solve_z_t<double>(a1.v, a2.v, a3.v, gas);
+        const double inv = -1.0 / ((3 * Z + 2 * a1.v) * Z + a2.v);
+        T r = (Z * Z * a1 + Z * a2 + a3) * inv;
+        r.v = Z;
+        return r;
+    } else {
+        return solve_z_val(a1, a2, a3, gas);
+    }
 }
 
 // Mixing sums with unscaled a: Ca = sum x_i x_j sqrt(a_i a_j) bic_ij, Cb = sum x b.

@@ -83,8 +83,11 @@ struct AsmArgs {
 __device__ __forceinline__ int nfields(int nc) { return 16 + 2 * nc; }
 
 // ---------------------------------------------------------------- EOS (T)
-template <class T>
-__device__ __forceinline__ T zroot(T a1, T a2, T a3, bool gas) {
+// Root of Z^3 + a1 Z^2 + a2 Z + a3 by the branches of Solve_Z
+// (Globals.cpp:212-264): the largest root for the gas, the smallest positive
+// one for the liquid.
+__device__ __forceinline__ double zroot(double a1, double a2, double a3, bool gas) {
+    using T = double;
     const T Q = (3.0 * a2 - a1 * a1) / 9.0;
     const T J = (9.0 * a1 * a2 - 27.0 * a3 - 2.0 * a1 * a1 * a1) / 54.0;
     const T D = Q * Q * Q + J * J;
@@ -118,6 +121,21 @@ __device__ __forceinline__ T zroot(T a1, T a2, T a3, bool gas) {
     return Z;
 }
 
+// Forward mode: the value from the branches above, the derivative from the
+// implicit function theorem on f(Z) = 0, dZ = -(Z^2 da1 + Z da2 + da3) / f'(Z),
+// as the reference's hand Jacobian takes dZ (Do_Cycle.cpp:438-616).
+// Differentiating the explicit formulas instead fails where they cancel: in
+// the one-root branch with Q -> 0, J - sqrt(D) rounds to exactly 0 and its
+// cube root has an infinite slope.
+template <int W>
+__device__ __forceinline__ DualV<W> zroot(const DualV<W>& a1, const DualV<W>& a2, const DualV<W>& a3, bool gas) {
+    const double Z = zroot(a1.v, a2.v, a3.v, gas);
+    const double inv = -1.0 / ((3.0 * Z + 2.0 * a1.v) * Z + a2.v);
+    DualV<W> r = (Z * Z * a1 + Z * a2 + a3) * inv;
+    r.v = Z;
+    return r;
+}
+
 template <int NC, class T>
 __device__ __forceinline__ void mixT(const AsmConsts& K, const T* x, T& Ca, T& Cb, T* t) {
     T u[NC];
@@ -146,7 +164,7 @@ __device__ __forceinline__ T zphase(const AsmConsts& K, const T* x, const T& P, 
     const bool srk = K.eos == OSR_EOS_SRK;
     const T U = srk ? B : 2.0 * B;
     const T W = srk ? T(0.0) : B;
-    return zroot<T>(-(1.0 + B - U), A - B * U - U - W * W, -(A * B - B * W * W - W * W), gas);
+    return zroot(-(1.0 + B - U), A - B * U - U - W * W, -(A * B - B * W * W - W * W), gas);
 }
 
 // ------------------------------------------------------- cell properties
@@ -1694,6 +1712,87 @@ __global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, c
     if (q == 0) bcols[bz] = d;
 #pragma unroll
     for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
+}
+
+// properties as duals from their values and a derivative column g[f * ncol]
+template <int NC>
+__device__ __forceinline__ void dual_props(const PropsT<NC, double>& pv, const double* g, int ncol,
+                                           PropsT<NC, DualV<1>>& p) {
+    auto F = [&](double v, int f) {
+        DualV<1> r;
+        r.v = v;
+        r.d[0] = g[f * ncol];
+        return r;
+    };
+    p.porpor = F(pv.porpor, 0); p.Pw = F(pv.Pw, 1); p.Po = F(pv.Po, 2); p.Pg = F(pv.Pg, 3);
+    p.bift = F(pv.bift, 4);
+    p.ro[0] = F(pv.ro[0], 5); p.ro[1] = F(pv.ro[1], 6); p.mu[0] = F(pv.mu[0], 7); p.mu[1] = F(pv.mu[1], 8);
+    p.gam[0] = F(pv.gam[0], 9); p.gam[1] = F(pv.gam[1], 10);
+    p.kr[0] = F(pv.kr[0], 11); p.kr[1] = F(pv.kr[1], 12); p.kr[2] = F(pv.kr[2], 13);
+    p.pcw = F(pv.pcw, 14); p.pcg = F(pv.pcg, 15);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        p.D[0][i] = F(pv.D[0][i], 16 + i);
+        p.D[1][i] = F(pv.D[1][i], 16 + NC + i);
+    }
+}
+
+// k_red_nb with one lane per (row block, slot) looping over the directions:
+// the dependency's values are loaded once, so the value half of every dual
+// operation is loop-invariant
+template <int NC>
+__global__ __launch_bounds__(256) void k_red_nb_loop(AsmArgs A, const double* dred, const double* S, const double* t,
+                                                     const int64_t* browptr, int32_t* bcols, double* bvals,
+                                                     double* rpart) {
+    using D1 = DualV<1>;
+    constexpr int NLM = NC + 3, NP = NC + 1, NF = JacShape<NC>::NF;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (int64_t)2 * A.n * 7) return;
+    const int slot = 1 + (int)(item % 7);
+    const int blk = (int)(item / 7);
+    const int n = A.n, k = blk / n, c = blk % n;
+    if (A.phi[k * n + c] < 1e-10) return;
+    const Geo G = geom(A, c);
+    const int d = dep_block(A, k, G, c, slot);
+    if (d < 0) return;
+    const int dk = d / n, dc = d % n;
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    PropsT<NC, double> p;
+    load_props<NC>(A, k, c, p);
+    VarsT<NC, double> vd;
+    load_vars<NC>(A, dk, dc, vd);
+    PropsT<NC, double> pd;
+    load_props<NC>(A, dk, dc, pd);
+    const int dnl = A.phi[dk * n + dc] < 1e-10 ? 0 : nloc(vd.lab, NC);
+    const double* Sd = S + (int64_t)d * NLM * NP;
+    const double* td = t + (int64_t)d * NLM;
+    const double* gd = dred + (int64_t)d * NF * (NP + 1);
+    const int64_t bz = browptr[blk] + dep_rank(A, k, G, c, d);
+    bcols[bz] = d;
+    const FaceGeo FG = face_geo(A, G, slot - 1);
+#pragma unroll 1
+    for (int q = 0; q <= NP; q++) {
+        VarsT<NC, D1> sv;
+        seed_dir<NC>(vd, Sd, td, dnl, q, A.pcoeff, sv);
+        PropsT<NC, D1> sp;
+        dual_props<NC>(pd, gd + q, NP + 1, sp);
+        RowAcc<NC, D1> R;
+        if (slot == 7) {
+            if (k == 0) transfer_piece<NC, double, D1>(A, 0, G, c, v, p, sv, sp, R);
+            else transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, v, p, R);
+        } else {
+            face_piece<NC, double, D1>(A, k, G, FG, slot - 1, c, dc, v, p, sv, sp, R);
+        }
+        if (q == NP) {
+            double* o = rpart + ((int64_t)blk * 8 + slot) * NP;
+#pragma unroll
+            for (int i = 0; i < NP; i++) o[i] = R.bal[i].d[0];
+        } else {
+#pragma unroll
+            for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
+        }
+    }
 }
 
 template <int NC>
