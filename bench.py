@@ -1,16 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the per-cell Newton hot path on one MI355X.
 
-Workload (the batched-flash configuration): 8 pseudo-components, PR EOS,
-2,000,000 independent cells, T ~ U(330, 420) K, z ~ Dirichlet(1.5), p placed
-at +/-15 % of each cell's saturation pressure, so convergence is uneven
-(single-phase cells stop after 2 passes, two-phase ones after tens to 500).
-A step is one flash of every cell (SuccFalsh semantics, FP64) with inputs
-already resident in HBM.  `value` is cells/s of the persistent work-queue
-form (FP64 passes, slow cells finished by the lane-group kernel); the plain
-one-lane-per-cell FP64 form and the mixed FP32/FP64-Newton form are timed
-beside it, and the assembly leg times residual + Jacobian per Newton
-iteration on the 64x64x8 dual-porosity grid.
+Headline workload (the batched-realization configuration): 16 dual-porosity
+grids with nx, ny ~ U{32..160}, nz ~ U{8..32} plus one 256x256x32 grid, 10
+pseudo-components (PR), rock as the 10x10x1 workload, p ~ U(8, 20) MPa,
+state = flash of (p, z ~ Dirichlet(2)).  A step is one Newton-iteration
+evaluation of every matrix and fracture cell of every grid, replayed as one
+hipGraph: batched flash of every cell, property sweep + residual, the
+phase-appearance test (PhaseCtrl) and the Jacobian.  `value` is cells/s
+(both continua counted) of the mixed/analytic path -- mixed FP32/FP64 flash
+and the reduced (nc+1)^2 block system assembled straight from the state;
+the plain FP64 path -- one-lane-per-cell FP64 flash and the reference-layout
+CSR Jacobian -- is timed beside it on the 256x256x32 grid (4,194,304 cells).
+
+Secondary legs: the batched-flash configuration (8 pseudo-components, 2M
+independent cells at +/-15 % of Psat; plain, persistent and mixed forms,
+parity against the oracle) and the 64x64x8 differentiation configuration
+(permeability adjoint, flash adjoint).  The CPU baseline is the oracle (FP64
+C/C++ restatement) on bounded samples of the same workloads.
 
 Prints ONE JSON line (rank 0).  Only one GPU is used; --gpus > 1 exits.
 """
@@ -64,9 +71,11 @@ def read_traffic(path):
 
 
 def assembly_leg(rt, synth, torch, args):
-    """Residual + Jacobian assembly per Newton iteration on the 64x64x8
-    dual-porosity grid with 6 components (65,536 cells per continuum):
-    property sweep, residual rows and the reference-layout CSR Jacobian."""
+    """The differentiation workload: 64x64x8 dual-porosity grid with 6
+    components (65,536 cells per continuum): residual + reference-layout CSR
+    Jacobian per Newton iteration, the reduced system (from the CSR and
+    straight from the state), the permeability adjoint and the flash
+    adjoint."""
     fl = synth.fluid_pseudo(6, synth.PR, 3)
     ctx = rt.Context(fl)
 
@@ -119,6 +128,8 @@ def assembly_leg(rt, synth, torch, args):
     red = {}
     ms_red = timed(lambda: red.update(r=asm.reduced(d, F, rp, vals)), steps)
     nnzb = int(red["r"][1].shape[0])
+    # the same reduced system straight from the state (no CSR)
+    ms_red_direct = timed(lambda: asm.reduced_direct(d, F, out=red["r"]), steps)
     # adjoint of the residual w.r.t. log-permeability (cotangent F)
     ms_perm = timed(lambda: asm.perm_vjp(d, F), steps)
     # flash adjoint over the grid's cells (feed: the fracture liquid composition)
@@ -141,33 +152,169 @@ def assembly_leg(rt, synth, torch, args):
         "algorithmic_bytes": state_b + out_b,
         "achieved_GBs": (state_b + out_b) / (ms * 1e-3) / 1e9,
         "hbm_peak_GBs": PEAK_HBM_GBS,
-        "reduced_bsr": {"ms": ms_red, "nnzb": nnzb, "block": f"{nc + 1}x{nc + 1}"},
+        "reduced_bsr": {"from_csr_ms": ms_red, "direct_ms": ms_red_direct, "nnzb": nnzb,
+                        "block": f"{nc + 1}x{nc + 1}"},
         "perm_adjoint_ms": ms_perm,
         "flash_adjoint": {"ms": ms_adj, "cells": n},
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cells", type=int, default=N_CELLS)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-assembly", action="store_true", help="skip the assembly leg")
-    ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="after the timed steps, copy a tensor of this many MiB once (a "
-                         "kernel of known HBM traffic that scales rocprofv3 FETCH_SIZE / "
-                         "WRITE_SIZE counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+NEWTON_NC = 10
+NEWTON_FLUID_SEED = 11
+NEWTON_SEED = 4
+NEWTON_BIG = (256, 256, 32)
 
-    import torch
-    from osiris_amd import runtime as rt, synth
 
+def newton_shapes(seed):
+    """Grid shapes of the batched-realization workload: 16 grids with
+    nx, ny ~ U{32..160}, nz ~ U{8..32}, then the 256x256x32 grid."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    shapes = [(int(g.integers(32, 161)), int(g.integers(32, 161)), int(g.integers(8, 33))) for _ in range(16)]
+    return shapes + [NEWTON_BIG]
+
+
+def reduced_bytes_model(nc, nblocks, nnzb):
+    """Algorithmic HBM bytes of one reduced-system assembly: read the state
+    (label, P, x, y, S: 8(2nc+4)+1 per block) and the static cell data
+    (porosity, 3 permeabilities, 6 face kA/h: 80 per block); write the
+    reduced blocks ((nc+1)^2 doubles + a 4-byte column each), the reduced
+    right-hand side (nc+1 per block) and the elimination factors S, t
+    ((nc+3)(nc+2) per block)."""
+    np1 = nc + 1
+    return (nblocks * (8 * (2 * nc + 4) + 1 + 80) + nnzb * (8 * np1 * np1 + 4) +
+            nblocks * 8 * (np1 + (nc + 3) * (nc + 2)))
+
+
+def time_graph(batch, torch, steps, warmup):
+    """Capture the batch, replay `warmup` times, then time `steps` replays
+    (HIP events on the replay stream, and the host clock)."""
+    batch.capture()
+    for _ in range(warmup):
+        batch.replay()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        batch.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return e0.elapsed_time(e1) / steps, wall * 1e3
+
+
+def newton_leg(rt, synth, torch, args, log):
+    """Flash + assembly per Newton iteration on the batched-realization
+    workload: the plain FP64 path and the mixed/analytic path on the
+    256x256x32 grid, then the mixed/analytic path on all 17 grids in one
+    graph (the headline)."""
+    from osiris_amd.batch import NewtonBatch
+    fl = synth.fluid_pseudo(NEWTON_NC, synth.PR, NEWTON_FLUID_SEED)
+    ctx = rt.Context(fl)
+
+    def flash_fn(T, P, z, sw):
+        out = rt.flash(ctx, T, P, z, sw=sw, mode="persistent")
+        return {k: v.cpu().numpy() for k, v in out.items()}
+
+    shapes = newton_shapes(NEWTON_SEED)
+    if args.newton_scale < 1:
+        f = args.newton_scale
+        shapes = [(max(4, int(nx * f)), max(4, int(ny * f)), max(2, int(nz * f))) for nx, ny, nz in shapes]
+    t0 = time.perf_counter()
+    cases = [synth.dual_porosity_case(fl, nx, ny, nz, 100 + i, flash_fn, p_range=(8e6, 20e6))
+             for i, (nx, ny, nz) in enumerate(shapes)]
+    log(f"newton: {len(cases)} grids generated in {time.perf_counter() - t0:.1f} s")
+    big = cases[-1]
+    nbig = 2 * big[0].n
+    ncells = sum(2 * g.n for g, _ in cases)
+    steps_big = max(2, args.steps // 4)
+
+    # plain FP64 path on the 256x256x32 grid: one-lane-per-cell FP64 flash,
+    # reference-layout CSR Jacobian
+    b = NewtonBatch(ctx, fl, [big], form="csr", flash_mode="plain")
+    ms_plain, wall_plain = time_graph(b, torch, steps_big, 1)
+    nnz_plain = b.nnz()
+    b.close()
+    del b
+    torch.cuda.empty_cache()
+    log(f"newton: plain {ms_plain:.1f} ms on the big grid")
+
+    # mixed/analytic path on the same grid, plus the reduced assembly alone
+    b = NewtonBatch(ctx, fl, [big], form="reduced", flash_mode="mixed")
+    ms_mixed_big, wall_mixed_big = time_graph(b, torch, steps_big, 1)
+    it = b.items[0]
+    stream = torch.cuda.current_stream()
+    reps = max(3, steps_big)
+    it["asm"].reduced_direct(it["d"], it["F"], out=it["red"])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        it["asm"].reduced_direct(it["d"], it["F"], out=it["red"])
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_red = e0.elapsed_time(e1) / reps
+    nnzb_big = int(it["red"][1].shape[0])
+    b.close()
+    del b, it
+    torch.cuda.empty_cache()
+    log(f"newton: mixed {ms_mixed_big:.1f} ms on the big grid, reduced assembly {ms_red:.1f} ms")
+
+    # all grids in one graph (headline)
+    b = NewtonBatch(ctx, fl, cases, form="reduced", flash_mode="mixed")
+    ms_all, wall_all = time_graph(b, torch, args.steps, args.warmup)
+    flash_stats = rt.flash_stats(ctx)
+    labels = b.fout["label"].cpu().numpy()
+    nnzb_all = sum(int(it["red"][1].shape[0]) for it in b.items)
+    b.close()
+    del b
+    torch.cuda.empty_cache()
+    log(f"newton: all grids {ms_all:.1f} ms")
+
+    red_bytes = reduced_bytes_model(NEWTON_NC, nbig, nnzb_big)
+    return {
+        "ncells": ncells, "ms_all": ms_all, "wall_ms_all": wall_all,
+        "detail": {
+            "workload": f"{len(cases) - 1} dual-porosity grids nx,ny~U{{32..160}}, nz~U{{8..32}} + "
+                        f"{'x'.join(map(str, shapes[-1]))}, {NEWTON_NC} components PR, one hipGraph per "
+                        "Newton-iteration evaluation: batched flash of every cell, property sweep + residual, "
+                        "PhaseCtrl, Jacobian",
+            "grids": [list(sh) for sh in shapes],
+            "cells": ncells,
+            "mixed_analytic_all_grids": {"ms_per_iter": ms_all, "cells_per_s": ncells / (ms_all * 1e-3),
+                                         "reduced_blocks": nnzb_all},
+            "big_grid": {
+                "cells": nbig,
+                "plain_fp64_csr": {"ms_per_iter": ms_plain, "cells_per_s": nbig / (ms_plain * 1e-3),
+                                   "jacobian_nnz": nnz_plain},
+                "mixed_analytic_reduced": {"ms_per_iter": ms_mixed_big,
+                                           "cells_per_s": nbig / (ms_mixed_big * 1e-3),
+                                           "reduced_blocks": nnzb_big},
+                "speedup_mixed_vs_plain": ms_plain / ms_mixed_big,
+                "reduced_assembly_ms": ms_red,
+            },
+            "frac_two_phase_after_flash": float((labels == 0).mean()),
+            "flash_stats": {k: int(v) for k, v in flash_stats.items()},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "reduced-system assembly of the 256x256x32 grid (osr_jacobian_bsr_reduced_direct_device: "
+                      "k_props, k_dprops, k_red_loc, k_red_local, k_red_dprops, k_red_own, k_red_nb, k_red_rhs)",
+            "achieved": red_bytes / (ms_red * 1e-3) / 1e9,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": red_bytes / (ms_red * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "algorithmic_bytes_per_launch": red_bytes,
+            "avg_launch_ms": ms_red,
+            "traffic": read_traffic(os.path.join(ROOT, "profiles", "reduced_assembly_pr10_256x256x32_pmc.txt")),
+        },
+    }
+
+
+def flash_leg(rt, synth, torch, args):
+    """The batched-flash workload: plain, persistent and mixed forms on 2M
+    cells (8 components, PR), flop-model roofline of the persistent form."""
     dev = torch.device("cuda:0")
     fl = synth.fluid_pseudo(NC, synth.PR, FLUID_SEED)
     ctx = rt.Context(fl)
@@ -177,6 +324,7 @@ def main():
     P = torch.tensor(cells.P, device=dev)
     z = torch.tensor(cells.z, device=dev)
     stream = torch.cuda.current_stream()
+    steps = max(3, args.steps // 2)
 
     def run(mode, out, steps, warmup):
         for _ in range(warmup):
@@ -185,26 +333,25 @@ def main():
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(steps)]
         t0 = time.perf_counter()
-        for s, e in ev:
-            s.record(stream)
+        for s_, e in ev:
+            s_.record(stream)
             rt.flash(ctx, T, P, z, mode=mode, out=out)
             e.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        kern = float(np.mean([s.elapsed_time(e) for s, e in ev])) * 1e-3
+        kern = float(np.mean([s_.elapsed_time(e) for s_, e in ev])) * 1e-3
         return wall / steps, kern
 
     out = rt.alloc_flash_out(NC, n)
-    sec_persist, kern_persist = run("persistent", out, args.steps, args.warmup)
+    sec_persist, kern_persist = run("persistent", out, steps, 1)
     iters = out["iters"].cpu().numpy()
     rr = out["rr_iters"].cpu().numpy()
     labels = out["label"].cpu().numpy()
     gpu_res = {k: out[k].cpu().numpy() for k in ("x", "y", "K", "L", "label", "iters")}
-
     out_plain = rt.alloc_flash_out(NC, n)
-    sec_plain, kern_plain = run("plain", out_plain, max(3, args.steps // 4), 1)
+    sec_plain, kern_plain = run("plain", out_plain, max(3, steps // 4), 1)
     out_mixed = rt.alloc_flash_out(NC, n)
-    sec_mixed, kern_mixed = run("mixed", out_mixed, args.steps, 1)
+    sec_mixed, kern_mixed = run("mixed", out_mixed, steps, 1)
     mixed_res = {k: out_mixed[k].cpu().numpy() for k in ("x", "y", "K", "L", "label", "iters")}
 
     if args.calib_copy_mb > 0:
@@ -216,33 +363,15 @@ def main():
 
     flops = flop_model(NC, iters, rr)
     achieved = flops / kern_persist / 1e12
-    cells_per_s = n / sec_persist
-
-    result = {
-        "metric": "flash+assembly cells/s per Newton iter, plain FP64 vs mixed/analytic @<=1e-8 err",
-        "value": cells_per_s,
-        "unit": "cells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": sec_persist * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded PCG64 generator; no reference fixtures exist)",
-        "config": {
-            "workload": "batched flash with uneven convergence: 8 pseudo-components, PR, "
-                        f"{n} independent cells, p within +/-15% of Psat, FP64 persistent "
-                        "work-queue kernel",
-            "n_cells": n, "nc": NC, "eos": "PR", "parallelism": "single GPU",
-        },
-        "paths": {
-            "persistent_fp64": {"cells_per_s": cells_per_s, "kernel_ms": kern_persist * 1e3},
-            "plain_fp64": {"cells_per_s": n / sec_plain, "kernel_ms": kern_plain * 1e3},
-            "mixed_fp32_fp64": {"cells_per_s": n / sec_mixed, "kernel_ms": kern_mixed * 1e3},
-            "speedup_persistent_vs_plain": sec_plain / sec_persist,
-        },
+    res = {
+        "workload": "batched flash with uneven convergence: 8 pseudo-components, PR, "
+                    f"{n} independent cells, p within +/-15% of Psat",
+        "n_cells": n,
+        "persistent_fp64": {"cells_per_s": n / sec_persist, "kernel_ms": kern_persist * 1e3},
+        "plain_fp64": {"cells_per_s": n / sec_plain, "kernel_ms": kern_plain * 1e3},
+        "mixed_fp32_fp64": {"cells_per_s": n / sec_mixed, "kernel_ms": kern_mixed * 1e3},
+        "speedup_persistent_vs_plain": sec_plain / sec_persist,
+        "speedup_mixed_vs_plain": sec_plain / sec_mixed,
         "workload_stats": {
             "mean_ss_passes": float(iters.mean()), "max_ss_passes": int(iters.max()),
             "frac_two_phase": float((labels == 0).mean()),
@@ -252,52 +381,172 @@ def main():
             "bound": "mfma",
             "bound_note": "FP64 vector ALU; MI355X FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s",
             "kernel": f"k_flash_persistent<{NC}> + k_flash_group<{NC}, 8> (one flash step)",
-            "achieved": achieved,
-            "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS,
-            "flops_per_launch": flops,
+            "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS, "flops_per_launch": flops,
             "avg_launch_ms": kern_persist * 1e3,
             "algorithmic_bytes_per_launch": bytes_model(NC, n),
             "traffic": read_traffic(os.path.join(ROOT, "profiles", "flash_persistent_pr8_2M_pmc.txt")),
         },
     }
+    return res, fl, cells, gpu_res, mixed_res
 
+
+def cpu_newton_sample(rt, synth, torch):
+    """CPU baseline of the Newton-iteration step: the oracle's flash of every
+    cell, PhaseCtrl of both continua, residual and reference-layout Jacobian
+    on a 16x16x8 sample grid of the same workload (10 components), one
+    thread (the oracle's assembly is a scalar restatement); the GPU's CSR
+    Jacobian and residual of the same sample checked against it."""
+    from oracle import oracle as O
+    from osiris_amd.batch import NewtonBatch
+    fl = synth.fluid_pseudo(NEWTON_NC, synth.PR, NEWTON_FLUID_SEED)
+
+    def oflash(T, P, z, sw):
+        r = O.flash_batch(fl, T, P, z, sw=sw)
+        L, Zl, Zg = r["L"], r["Zl"], r["Zg"]
+        nd = Zl * L / (Zl * L + Zg * (1 - L))
+        r["so"], r["sg"] = nd * (1 - sw), (1 - nd) * (1 - sw)
+        return r
+
+    grid, st = synth.dual_porosity_case(fl, 16, 16, 8, 99, oflash, p_range=(8e6, 20e6))
+    n = grid.n
+    T = np.full(2 * n, grid.T)
+    P = np.concatenate([st.Po_m, st.Po_f])
+    z = np.concatenate([st.z_m, st.z_f], axis=1)
+    t0 = time.perf_counter()
+    O.flash_batch(fl, T, P, z, nthreads=1)
+    pr = O.props(fl, grid, st)
+    for k, (lab, Po, x, y, S) in enumerate(((st.label_m, st.Po_m, st.x_m, st.y_m, st.S_m),
+                                            (st.label_f, st.Po_f, st.x_f, st.y_f, st.S_f))):
+        O.phase_ctrl(fl, np.full(n, grid.T), lab.copy(), Po, pr[k, :, 3].copy(), x.copy(), y.copy(), S.copy(),
+                     nthreads=1)
+    F_or = O.residual(fl, grid, st)
+    rp, cols, vals = O.jacobian(fl, grid, st)
+    cpu_s = time.perf_counter() - t0
+    # the GPU on the same sample (CSR form), against the oracle
+    ctx = rt.Context(fl)
+    b = NewtonBatch(ctx, fl, [(grid, st)], form="csr", flash_mode="plain")
+    b.run_eager()
+    torch.cuda.synchronize()
+    it = b.items[0]
+    Fg, vg = it["F"].cpu().numpy(), it["vals"].cpu().numpy()
+    same_pattern = bool(np.array_equal(it["rowptr"].cpu().numpy(), rp) and
+                        np.array_equal(it["cols"].cpu().numpy(), cols))
+    Fs = O.residual_scale(fl, grid, st)
+    fin = np.isfinite(F_or)
+    f_err = float((np.abs(Fg[fin] - F_or[fin]) / np.maximum(Fs[fin], 1e-14 * np.nanmax(Fs) + 1e-300)).max())
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    rowmax = np.zeros(len(rp) - 1)
+    np.maximum.at(rowmax, rows, np.abs(vals))
+    j_err = float((np.abs(vg - vals) / np.maximum(np.abs(vals), 1e-8 * rowmax[rows])).max())
+    b.close()
+    return {"value": 2 * n / cpu_s, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"16x16x8 dual-porosity grid of the same workload ({2 * n} cells, 10 components): oracle "
+                      "flash of every cell + PhaseCtrl + residual + reference-layout Jacobian, one thread",
+            "seconds": cpu_s}, {"jacobian_pattern_identical": same_pattern,
+                                "residual_max_row_scaled_err": f_err,
+                                "jacobian_max_rel_err": j_err}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=N_CELLS)
+    ap.add_argument("--newton-scale", type=float, default=1.0,
+                    help="scale the grid dimensions of the batched-realization workload (debugging)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-flash", action="store_true", help="skip the batched-flash leg")
+    ap.add_argument("--no-assembly", action="store_true", help="skip the 64x64x8 sensitivity leg")
+    ap.add_argument("--no-newton", action="store_true", help="skip the Newton-iteration leg")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the timed flash steps, copy a tensor of this many MiB once (a "
+                         "kernel of known HBM traffic that scales rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE counters)")
+    ap.add_argument("--quiet", action="store_true", help="no progress lines on stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from osiris_amd import runtime as rt, synth
+
+    def log(msg):
+        if not args.quiet:
+            print(msg, file=sys.stderr, flush=True)
+
+    result = {
+        "metric": "flash+assembly cells/s per Newton iter, plain FP64 vs mixed/analytic @<=1e-8 err",
+        "value": None, "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 generator; no reference fixtures exist)",
+    }
+    if not args.no_newton:
+        nl = newton_leg(rt, synth, torch, args, log)
+        result["value"] = nl["ncells"] / (nl["ms_all"] * 1e-3)
+        result["ms_per_step"] = nl["ms_all"]
+        result["wall_ms_per_step"] = nl["wall_ms_all"]
+        result["config"] = {"workload": nl["detail"]["workload"], "cells": nl["ncells"], "nc": NEWTON_NC,
+                            "eos": "PR", "parallelism": "single GPU"}
+        result["newton"] = nl["detail"]
+        result["roofline"] = nl["roofline"]
+    if not args.no_flash:
+        fres, fl, cells, gpu_res, mixed_res = flash_leg(rt, synth, torch, args)
+        result["flash_batched"] = fres
+        log("flash leg done")
+        if args.no_newton:
+            result["value"] = fres["persistent_fp64"]["cells_per_s"]
+            result["ms_per_step"] = fres["persistent_fp64"]["kernel_ms"]
+            result["config"] = {"workload": fres["workload"], "n_cells": args.cells, "nc": NC, "eos": "PR",
+                                "parallelism": "single GPU"}
+            result["roofline"] = fres["roofline"]
     if not args.no_assembly:
-        result["assembly"] = assembly_leg(rt, synth, torch, args)
+        result["sensitivities_64x64x8"] = assembly_leg(rt, synth, torch, args)
+        log("sensitivity leg done")
 
     if not args.no_cpu:
-        # CPU baseline: the oracle (FP64 C restatement of SuccFalsh) on the
-        # same 2M-cell step, all host threads up to 16, plus a 1-thread run
-        # on a 100k-cell sample.  Parity of the timed GPU step against it.
         from oracle import oracle as O
         threads = min(16, os.cpu_count() or 1)
-        t0 = time.perf_counter()
-        ref = O.flash_batch(fl, cells.T, cells.P, cells.z, nthreads=threads)
-        cpu_s = time.perf_counter() - t0
-        m = 100_000 if n > 100_000 else n
-        t0 = time.perf_counter()
-        O.flash_batch(fl, cells.T[:m], cells.P[:m], cells.z[:, :m], nthreads=1)
-        cpu1_s = time.perf_counter() - t0
-        def parity(res):
-            conv = (ref["iters"] < 500) & (res["iters"] < 500)
-            errs = {}
-            for k in ("x", "y", "K", "L"):
-                fin = np.isfinite(ref[k]) & np.isfinite(res[k])
-                e = np.abs(res[k] - ref[k]) / np.maximum(np.abs(ref[k]), 1e-12)
-                e = np.where(fin, e, 0.0)
-                errs[k] = float(e[..., conv].max())
-            return {"label_mismatches": int((res["label"] != ref["label"]).sum()),
-                    "max_rel_err_converged_cells": errs,
-                    "cells_capped_either_side": int((~conv).sum())}
-        result["cpu_baseline"] = {
-            "value": n / cpu_s, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"full {n}-cell step, FP64 C oracle with OpenMP over cells",
-            "single_core_cells_per_s": m / cpu1_s,
-            "single_core_sample": f"first {m} cells, 1 thread",
-        }
-        result["parity_vs_oracle"] = {"persistent_fp64": parity(gpu_res),
-                                      "mixed_fp32_fp64": parity(mixed_res)}
+        base = {}
+        if not args.no_newton:
+            base, par = cpu_newton_sample(rt, synth, torch)
+            result.setdefault("parity_vs_oracle", {})["newton_sample_csr"] = par
+            log("cpu newton sample done")
+        if not args.no_flash:
+            n = args.cells
+            t0 = time.perf_counter()
+            ref = O.flash_batch(fl, cells.T, cells.P, cells.z, nthreads=threads)
+            cpu_s = time.perf_counter() - t0
+            m = 100_000 if n > 100_000 else n
+            t0 = time.perf_counter()
+            O.flash_batch(fl, cells.T[:m], cells.P[:m], cells.z[:, :m], nthreads=1)
+            cpu1_s = time.perf_counter() - t0
+
+            def parity(res):
+                conv = (ref["iters"] < 500) & (res["iters"] < 500)
+                errs = {}
+                for k in ("x", "y", "K", "L"):
+                    fin = np.isfinite(ref[k]) & np.isfinite(res[k])
+                    e = np.abs(res[k] - ref[k]) / np.maximum(np.abs(ref[k]), 1e-12)
+                    e = np.where(fin, e, 0.0)
+                    errs[k] = float(e[..., conv].max())
+                return {"label_mismatches": int((res["label"] != ref["label"]).sum()),
+                        "max_rel_err_converged_cells": errs,
+                        "cells_capped_either_side": int((~conv).sum())}
+            result["flash_batched"]["cpu_baseline"] = {
+                "value": n / cpu_s, "unit": "cells/s", "cores": threads, "kind": "port",
+                "sample": f"full {n}-cell step, FP64 C oracle with OpenMP over cells",
+                "single_core_cells_per_s": m / cpu1_s, "single_core_sample": f"first {m} cells, 1 thread"}
+            result.setdefault("parity_vs_oracle", {})["flash_persistent_fp64"] = parity(gpu_res)
+            result["parity_vs_oracle"]["flash_mixed_fp32_fp64"] = parity(mixed_res)
+        if base:
+            result["cpu_baseline"] = base
+        elif "cpu_baseline" in result.get("flash_batched", {}):
+            result["cpu_baseline"] = result["flash_batched"]["cpu_baseline"]
     print(json.dumps(result))
 
 

@@ -931,8 +931,6 @@ __global__ __launch_bounds__(256) void k_dprops(AsmArgs A, double* dp) {
     for (int f = 0; f < NF; f++) o[f * MAXC] = prop_field<NC, D1>(sp, f).d[0];
 }
 
-// properties of block blk as duals: values from the property table, the
-// derivative along direction j from a workspace laid out [block][field][ncol]
 template <int NC>
 __device__ __forceinline__ void load_props_dual(const AsmArgs& A, const double* dp, int blk, int j,
                                                 PropsT<NC, DualV<1>>& p, int ncol = JacShape<NC>::MAXC) {
@@ -1236,6 +1234,21 @@ __device__ __forceinline__ int red_sec_col(int lab, int nc, int i) {
     return i == 0 ? nc - 1 : nc + 2;                                 // comp_{nc-1}, So|Sg
 }
 
+// role of unknown column j of a block: primary index (>= 0), or -1 - the
+// secondary index (the inverse of red_prim_col / red_sec_col)
+__device__ __forceinline__ int red_role(int lab, int nc, int j) {
+    if (lab == 0) {
+        if (j < nc - 2) return j;
+        if (j < 2 * nc) return -1 - (j - (nc - 2));
+        if (j <= 2 * nc + 2) return nc - 2 + (j - 2 * nc);
+        return -1 - (nc + 2);
+    }
+    if (j < nc - 1) return j;
+    if (j == nc - 1) return -1;
+    if (j <= nc + 1) return nc - 1 + (j - nc);
+    return -2;
+}
+
 // entry (row r of block blk, column col of dependency block d) of the CSR;
 // sm, sw: the dependency's segment start in the balance rows and the water
 // row (seg_start, kinds 0 and 1)
@@ -1255,77 +1268,69 @@ __device__ __forceinline__ double csr_at(const AsmArgs& A, int blk, int lab, int
     return d == blk ? vals[rowptr[rb + r] + col] : 0.0;   // local rows: own block only
 }
 
-// Gauss elimination with partial pivoting of one block's local system,
-// held in a workspace laid out [entry][block] so the lanes' accesses
-// coalesce: rows 0..nl-1, columns [secondary 0..nl-1 | primary 0..NP-1 | F].
-// Leaves S (nl x NP) and t (nl) of ds = -S dp - t in So, to.
+// Gauss-Jordan elimination of one block's local system by a group of
+// RED_GROUP = 32 lanes of a wave: lane j < block size holds column j of the
+// local rows (the block's unknown j), lane 31 the residual column.  The
+// pivots run over the secondary columns in order; each is chosen by partial
+// pivoting (largest magnitude at or below the diagonal, the first on ties)
+// by the lane holding that column and broadcast with a cross-lane read, so
+// every lane updates its own column in registers.  Afterwards the primary
+// lanes hold S = J_ls^-1 J_lp column by column and lane 31 holds
+// t = J_ls^-1 F_l.  nl and lab are uniform over the group.
+constexpr int RED_GROUP = 32;
+
 template <int NC>
-__device__ __forceinline__ void local_eliminate(double* W, int64_t nb, int blk, int nl, double* So, double* to) {
-    constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
-#define WM(r, q) W[((int64_t)(r) * NW + (q)) * nb + blk]
-    const int nr = nl + NP + 1;
-    for (int q = 0; q < nl; q++) {
+__device__ __forceinline__ void group_eliminate(double (&col)[NC + 3], int lab, int nl, int gbase) {
+    constexpr int NLM = NC + 3;
+#pragma unroll
+    for (int q = 0; q < NLM; q++) {
+        if (q >= nl) break;
+        const int P = gbase + red_sec_col(lab, NC, q);
         int piv = q;
-        double best = fabs(WM(q, q));
-        for (int r = q + 1; r < nl; r++) {
-            const double v = fabs(WM(r, q));
-            if (v > best) { best = v; piv = r; }
-        }
-        if (piv != q)
-            for (int cc = q; cc < nr; cc++) {
-                const double tmp = WM(q, cc);
-                WM(q, cc) = WM(piv, cc);
-                WM(piv, cc) = tmp;
+        double best = fabs(col[q]);
+#pragma unroll
+        for (int r = q + 1; r < NLM; r++) {
+            const double v = fabs(col[r]);
+            if (r < nl && v > best) {
+                best = v;
+                piv = r;
             }
-        const double ip = 1.0 / WM(q, q);
-        for (int r = q + 1; r < nl; r++) {
-            const double f = WM(r, q) * ip;
-            if (f == 0) continue;
-            for (int cc = q + 1; cc < nr; cc++) WM(r, cc) -= f * WM(q, cc);
         }
+        piv = __shfl(piv, P, 64);
+        const double cq = col[q];
+        double cp = cq;
+#pragma unroll
+        for (int r = q + 1; r < NLM; r++)
+            if (r == piv) cp = col[r];
+#pragma unroll
+        for (int r = q + 1; r < NLM; r++)
+            if (r == piv) col[r] = cq;
+        col[q] = cp;
+        double f[NLM];
+#pragma unroll
+        for (int r = 0; r < NLM; r++) f[r] = __shfl(col[r], P, 64);
+        const double cn = col[q] * (1.0 / f[q]);
+#pragma unroll
+        for (int r = 0; r < NLM; r++)
+            if (r != q && r < nl) col[r] -= f[r] * cn;
+        col[q] = cn;
     }
-    for (int q = nl - 1; q >= 0; q--) {
-        const double ip = 1.0 / WM(q, q);
-        for (int cc = nl; cc < nr; cc++) {
-            double v = WM(q, cc);
-            for (int r = q + 1; r < nl; r++) v -= WM(q, r) * WM(r, cc);
-            WM(q, cc) = v * ip;
-        }
-    }
-    for (int i = 0; i < nl; i++) {
-        for (int q = 0; q < NP; q++) So[i * NP + q] = WM(i, nl + q);
-        to[i] = WM(i, nl + NP);
-    }
-#undef WM
 }
 
-// one lane per block: S, t of the local elimination, the local rows read
-// from the full CSR
+// S, t of one block from the group's columns (zero beyond the block's nl rows)
 template <int NC>
-__global__ __launch_bounds__(256) void k_reduce_local(AsmArgs A, const double* F, const int64_t* rowptr,
-                                                      const double* vals, double* W, double* S, double* t) {
-    constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
-    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = A.n;
-    if (blk >= 2 * n) return;
-    const int64_t nb = 2 * (int64_t)n;
-    const int k = blk / n, c = blk % n;
-    const int lab = A.lab[k][c];
-    double* So = S + (int64_t)blk * NLM * NP;
-    double* to = t + (int64_t)blk * NLM;
-    for (int i = 0; i < NLM * NP; i++) So[i] = 0;
-    for (int i = 0; i < NLM; i++) to[i] = 0;
-    if (A.phi[k * n + c] < 1e-10) return;   // identity rows: S = 0, t = 0
-    const int nl = nloc(lab, NC);
-    const int64_t rb = A.boff[blk];
-    for (int i = 0; i < nl; i++) {
-        const int row = loc_row(lab, NC, i);
-        double* w = W + (int64_t)i * NW * nb + blk;
-        for (int j = 0; j < nl; j++) w[(int64_t)j * nb] = vals[rowptr[rb + row] + red_sec_col(lab, NC, j)];
-        for (int q = 0; q < NP; q++) w[(int64_t)(nl + q) * nb] = vals[rowptr[rb + row] + red_prim_col(lab, NC, q)];
-        w[(int64_t)(nl + NP) * nb] = F[rb + row];
+__device__ __forceinline__ void group_write_st(const double (&col)[NC + 3], int lab, int nl, int j, int bs,
+                                               double* Sb, double* tb) {
+    constexpr int NLM = NC + 3, NP = NC + 1;
+    if (j == RED_GROUP - 1) {
+#pragma unroll
+        for (int r = 0; r < NLM; r++) tb[r] = r < nl ? col[r] : 0.0;
+    } else if (j < bs) {
+        const int role = red_role(lab, NC, j);
+        if (role >= 0)
+#pragma unroll
+            for (int r = 0; r < NLM; r++) Sb[r * NP + role] = r < nl ? col[r] : 0.0;
     }
-    local_eliminate<NC>(W, nb, blk, nl, So, to);
 }
 
 // block counts of the reduced rows (zero-porosity rows keep only their own block)
@@ -1451,8 +1456,8 @@ __global__ void k_reduce_expand(AsmArgs A, const double* S, const double* t, con
 // and the right-hand-side correction sum_d J_ms,d t_d the one along
 //   w   = sum_s t_d[s] e_{secondary s}                        (q = NP).
 //   k_red_loc    one lane per (block, unknown j): the local rows' column j
-//                (own_piece in forward mode) into the elimination workspace;
-//   k_red_elim   one lane per block: S, t;
+//                (own_piece in forward mode);
+//   k_red_local  one lane group per block: the columns eliminated to S, t;
 //   k_red_dprops one lane per (block, direction q <= NP): the property
 //                derivatives along v_q / w, combined from the per-unknown
 //                ones (the property sweep is linear in the seed);
@@ -1460,21 +1465,6 @@ __global__ void k_reduce_expand(AsmArgs A, const double* S, const double* t, con
 //                block / of dependency slot 1..7, written to the reduced
 //                block's column q, or (q = NP) to a per-slot partial sum;
 //   k_red_rhs    one lane per block: -F_m plus the slot partials.
-
-// role of unknown column j of a block: primary index (>= 0), or -1 - the
-// secondary index (the inverse of red_prim_col / red_sec_col)
-__device__ __forceinline__ int red_role(int lab, int nc, int j) {
-    if (lab == 0) {
-        if (j < nc - 2) return j;
-        if (j < 2 * nc) return -1 - (j - (nc - 2));
-        if (j <= 2 * nc + 2) return nc - 2 + (j - 2 * nc);
-        return -1 - (nc + 2);
-    }
-    if (j < nc - 1) return j;
-    if (j == nc - 1) return -1;
-    if (j <= nc + 1) return nc - 1 + (j - nc);
-    return -2;
-}
 
 // the block's unknowns seeded along reduced direction q (see above); nl is
 // the number of secondaries in play (0 for a zero-porosity block)
@@ -1515,10 +1505,13 @@ __device__ __forceinline__ void seed_dir(const VarsT<NC, double>& v, const doubl
     if (v.lab != 1) mark(out.S[2], 1.0);
 }
 
+// one lane per (block, unknown j): column j of the block's local rows
+// (fugacity and constraint rows) by forward mode of its own piece, into Wl
+// laid out [block][column][local row] (a lane's column is contiguous)
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, const double* dp, const double* F, double* W) {
+__global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, const double* dp, double* Wl) {
     using D1 = DualV<1>;
-    constexpr int MAXC = JacShape<NC>::MAXC, NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
+    constexpr int MAXC = JacShape<NC>::MAXC;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= (int64_t)2 * A.n * MAXC) return;
     const int j = (int)(item % MAXC);
@@ -1537,31 +1530,60 @@ __global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, const double* dp, co
     RowAcc<NC, D1> R;
     own_piece<NC, D1>(A, k, c, G, sv, sp, R);
     const int nl = nloc(lab, NC);
-    const int role = red_role(lab, NC, j);
-    const int wc = role < 0 ? -1 - role : nl + role;
-    const int64_t nb = 2 * (int64_t)n;
+    double* w = Wl + ((int64_t)blk * MAXC + j) * (NC + 3);
 #pragma unroll
-    for (int i = 0; i < NC + 3; i++)
-        if (i < nl) W[((int64_t)i * NW + wc) * nb + blk] = R.loc[i].d[0];
-    if (j == 0) {
-        const int64_t rb = A.boff[blk];
-        for (int i = 0; i < nl; i++) W[((int64_t)i * NW + nl + NP) * nb + blk] = F[rb + loc_row(lab, NC, i)];
-    }
+    for (int r = 0; r < NC + 3; r++)
+        if (r < nl) w[r] = R.loc[r].d[0];
 }
 
-template <int NC>
-__global__ __launch_bounds__(256) void k_red_elim(AsmArgs A, double* W, double* S, double* t) {
+// One group of RED_GROUP lanes per block: each lane loads its column of the
+// local rows -- from the full CSR (CSR = true) or from the columns k_red_loc
+// wrote (Wl) -- and the group eliminates (group_eliminate) to S, t.
+// Zero-porosity blocks (identity rows) get S = 0, t = 0.
+template <int NC, bool CSR>
+__global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, const double* F,
+                                                   const int64_t* rowptr, const double* vals, double* S,
+                                                   double* t) {
     constexpr int NLM = NC + 3, NP = NC + 1;
-    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    static_assert(2 * NC + 4 < RED_GROUP, "a block's unknowns and its residual column fit one lane group");
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = (int)(item / RED_GROUP);
+    const int j = (int)(item % RED_GROUP);
     const int n = A.n;
-    if (blk >= 2 * n) return;
+    if (blk >= 2 * n) return;                    // whole groups leave together
+    const int gbase = (int)(threadIdx.x & 63) & ~(RED_GROUP - 1);
     const int k = blk / n, c = blk % n;
-    double* So = S + (int64_t)blk * NLM * NP;
-    double* to = t + (int64_t)blk * NLM;
-    for (int i = 0; i < NLM * NP; i++) So[i] = 0;
-    for (int i = 0; i < NLM; i++) to[i] = 0;
-    if (A.phi[k * n + c] < 1e-10) return;   // identity rows: S = 0, t = 0
-    local_eliminate<NC>(W, 2 * (int64_t)n, blk, nloc(A.lab[k][c], NC), So, to);
+    const int lab = A.lab[k][c];
+    const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
+    double* Sb = S + (int64_t)blk * NLM * NP;
+    double* tb = t + (int64_t)blk * NLM;
+    double col[NLM];
+#pragma unroll
+    for (int r = 0; r < NLM; r++) col[r] = 0.0;
+    if (A.phi[k * n + c] < 1e-10) {
+        group_write_st<NC>(col, lab, 0, j, bs, Sb, tb);
+        return;
+    }
+    const int nl = nloc(lab, NC);
+    const int64_t rb = A.boff[blk];
+    if (j == RED_GROUP - 1) {
+#pragma unroll
+        for (int r = 0; r < NLM; r++)
+            if (r < nl) col[r] = F[rb + loc_row(lab, NC, r)];
+    } else if (j < bs) {
+        if (CSR) {
+#pragma unroll
+            for (int r = 0; r < NLM; r++)
+                if (r < nl) col[r] = vals[rowptr[rb + loc_row(lab, NC, r)] + j];
+        } else {
+            const double* w = Wl + ((int64_t)blk * JacShape<NC>::MAXC + j) * NLM;
+#pragma unroll
+            for (int r = 0; r < NLM; r++)
+                if (r < nl) col[r] = w[r];
+        }
+    }
+    group_eliminate<NC>(col, lab, nl, gbase);
+    group_write_st<NC>(col, lab, nl, j, bs, Sb, tb);
 }
 
 template <int NC>
@@ -1655,9 +1677,9 @@ __global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, 
         else transfer_piece<NC, double, D1>(A, 1, G, c, vx, px, sv, sp, R);
     }
     if (q == NP) {
-        double* o = rpart + (int64_t)blk * 8 * NP;
+        const int64_t nb = 2 * (int64_t)n;
 #pragma unroll
-        for (int i = 0; i < NP; i++) o[i] = R.bal[i].d[0];
+        for (int i = 0; i < NP; i++) rpart[(int64_t)i * nb + blk] = R.bal[i].d[0];
         return;
     }
     const int64_t bz = b0 + dep_rank(A, k, G, c, blk);
@@ -1703,9 +1725,9 @@ __global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, c
         face_piece<NC, double, D1>(A, k, G, FG, slot - 1, c, dc, v, p, sv, sp, R);
     }
     if (q == NP) {
-        double* o = rpart + ((int64_t)blk * 8 + slot) * NP;
+        const int64_t nb = 2 * (int64_t)n;
 #pragma unroll
-        for (int i = 0; i < NP; i++) o[i] = R.bal[i].d[0];
+        for (int i = 0; i < NP; i++) rpart[((int64_t)slot * NP + i) * nb + blk] = R.bal[i].d[0];
         return;
     }
     const int64_t bz = browptr[blk] + dep_rank(A, k, G, c, d);
@@ -1810,11 +1832,13 @@ __global__ void k_red_rhs(AsmArgs A, const double* F, const double* rpart, doubl
     const Geo G = geom(A, c);
     const int r0 = A.lab[k][c] == 0 ? NC : 0;
     const int64_t rb = A.boff[blk];
-    const double* rp = rpart + (int64_t)blk * 8 * NP;
+    const int64_t nb = 2 * (int64_t)n;
+    bool has[8];
+    for (int sl = 0; sl < 8; sl++) has[sl] = dep_block(A, k, G, c, sl) >= 0;
     for (int i = 0; i < NP; i++) {
         double v = -F[rb + r0 + i];
         for (int sl = 0; sl < 8; sl++)
-            if (dep_block(A, k, G, c, sl) >= 0) v += rp[sl * NP + i];
+            if (has[sl]) v += rpart[((int64_t)sl * NP + i) * nb + blk];
         o[i] = v;
     }
 }
@@ -1846,11 +1870,11 @@ struct osr_asm {
     double* dprops = nullptr;      // Jacobian workspace [2n][NF][2nc+4]
     double* perm_f = nullptr;      // [3n] fracture permeabilities (permeability adjoint)
     double* pslots = nullptr;      // permeability adjoint face slots [2][2][3][n]
-    double* rwork = nullptr;       // reduced-system local elimination workspace
     int64_t* bcnt = nullptr;       // [2n+1] block counts of the reduced rows
+    double* wloc = nullptr;        // direct reduction: local-row columns [2n][2nc+4][nc+3]
     double* dred = nullptr;        // direct reduction: property derivatives along the reduced directions
-    double* rpart = nullptr;       // direct reduction: per-slot right-hand-side partials [2n][8][nc+1]
-    bool ws_external = false;      // dprops, rwork, dred, rpart live in a caller's workspace
+    double* rpart = nullptr;       // direct reduction: per-slot right-hand-side partials [8][nc+1][2n]
+    bool ws_external = false;      // dprops, wloc, dred, rpart live in a caller's workspace
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     int64_t nrows = -1;
@@ -1983,11 +2007,10 @@ int reduced_nc(osr_asm* a, const osr_state_dev* s, const double* F, const int64_
                const int64_t* browptr, int32_t* bcols, double* bvals, double* brhs, double* S, double* t,
                hipStream_t st) {
     osr::AsmArgs A = make_args(a, s);
-    constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
+    constexpr int NP = NC + 1;
     const int64_t nb = 2 * (int64_t)a->n;
-    if (!a->rwork) OSR_HIP(hipMalloc((void**)&a->rwork, (size_t)NLM * NW * nb * sizeof(double)));
-    hipLaunchKernelGGL(osr::k_reduce_local<NC>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, A, F, rowptr,
-                       vals, a->rwork, S, t);
+    hipLaunchKernelGGL((osr::k_red_local<NC, true>), dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256),
+                       0, st, A, (const double*)nullptr, F, rowptr, vals, S, t);
     const int64_t g = (nb * 8 * NP + 255) / 256;
     hipLaunchKernelGGL(osr::k_reduce_blocks<NC>, dim3((unsigned)g), dim3(256), 0, st, A, F, rowptr, vals,
                        (const double*)S, (const double*)t, browptr, bcols, bvals, brhs);
@@ -2001,10 +2024,10 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     int rc = run_props<NC>(a, A, st);
     if (rc) return rc;
     constexpr int NF = osr::JacShape<NC>::NF, MAXC = osr::JacShape<NC>::MAXC;
-    constexpr int NLM = NC + 3, NP = NC + 1, NW = NLM + NP + 1;
+    constexpr int NP = NC + 1;
     const int64_t nb = 2 * (int64_t)a->n;
     if (!a->dprops) OSR_HIP(hipMalloc((void**)&a->dprops, (size_t)nb * NF * MAXC * sizeof(double)));
-    if (!a->rwork) OSR_HIP(hipMalloc((void**)&a->rwork, (size_t)NLM * NW * nb * sizeof(double)));
+    if (!a->wloc) OSR_HIP(hipMalloc((void**)&a->wloc, (size_t)nb * MAXC * (NC + 3) * sizeof(double)));
     if (!a->dred) OSR_HIP(hipMalloc((void**)&a->dred, (size_t)nb * NF * (NP + 1) * sizeof(double)));
     if (!a->rpart) OSR_HIP(hipMalloc((void**)&a->rpart, (size_t)nb * 8 * NP * sizeof(double)));
     const unsigned g1 = (unsigned)((nb + 255) / 256);
@@ -2012,8 +2035,9 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     const unsigned gq = (unsigned)((nb * (NP + 1) + 255) / 256);
     const unsigned gn = (unsigned)((nb * 7 * (NP + 1) + 255) / 256);
     hipLaunchKernelGGL(osr::k_dprops<NC>, dim3(gc), dim3(256), 0, st, A, a->dprops);
-    hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3(gc), dim3(256), 0, st, A, (const double*)a->dprops, F, a->rwork);
-    hipLaunchKernelGGL(osr::k_red_elim<NC>, dim3(g1), dim3(256), 0, st, A, a->rwork, S, t);
+    hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3(gc), dim3(256), 0, st, A, (const double*)a->dprops, a->wloc);
+    hipLaunchKernelGGL((osr::k_red_local<NC, false>), dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256),
+                       0, st, A, (const double*)a->wloc, F, (const int64_t*)nullptr, (const double*)nullptr, S, t);
     hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dprops,
                        (const double*)S, (const double*)t, a->dred);
     hipLaunchKernelGGL(osr::k_red_own<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, (const double*)S,
@@ -2179,7 +2203,7 @@ int osr_asm_destroy(osr_asm* a) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (!a->ws_external) {
-        void* ws[] = {a->dprops, a->rwork, a->dred, a->rpart};
+        void* ws[] = {a->dprops, a->wloc, a->dred, a->rpart};
         for (void* p : ws)
             if (p) hipFree(p);
     }
@@ -2187,11 +2211,11 @@ int osr_asm_destroy(osr_asm* a) {
     return OSR_OK;
 }
 
-// Jacobian workspace layout: [dprops | rwork | dred | rpart], 256-byte aligned
+// Jacobian workspace layout: [dprops | wloc | dred | rpart], 256-byte aligned
 static size_t ws_layout(const osr_asm* a, size_t off[4]) {
     const size_t nb = 2 * (size_t)a->n, nc = (size_t)a->nc;
-    const size_t NF = 16 + 2 * nc, MAXC = 2 * nc + 4, NLM = nc + 3, NP = nc + 1, NW = NLM + NP + 1;
-    const size_t sz[4] = {nb * NF * MAXC * 8, NLM * NW * nb * 8, nb * NF * (NP + 1) * 8, nb * 8 * NP * 8};
+    const size_t NF = 16 + 2 * nc, MAXC = 2 * nc + 4, NP = nc + 1;
+    const size_t sz[4] = {nb * NF * MAXC * 8, nb * MAXC * (nc + 3) * 8, nb * NF * (NP + 1) * 8, nb * 8 * NP * 8};
     size_t at = 0;
     for (int i = 0; i < 4; i++) {
         off[i] = at;
@@ -2219,19 +2243,19 @@ int osr_asm_set_workspace(osr_asm* a, void* ptr, size_t bytes) {
     const size_t need = ws_layout(a, off);
     if (ptr && (bytes < need || ((uintptr_t)ptr & 255))) return OSR_E_ARG;
     if (!a->ws_external) {
-        void* ws[] = {a->dprops, a->rwork, a->dred, a->rpart};
+        void* ws[] = {a->dprops, a->wloc, a->dred, a->rpart};
         for (void* p : ws)
             if (p) OSR_HIP(hipFree(p));
     }
     if (ptr) {
         char* b = (char*)ptr;
         a->dprops = (double*)(b + off[0]);
-        a->rwork = (double*)(b + off[1]);
+        a->wloc = (double*)(b + off[1]);
         a->dred = (double*)(b + off[2]);
         a->rpart = (double*)(b + off[3]);
         a->ws_external = true;
     } else {
-        a->dprops = a->rwork = a->dred = a->rpart = nullptr;
+        a->dprops = a->wloc = a->dred = a->rpart = nullptr;
         a->ws_external = false;
     }
     return OSR_OK;
