@@ -879,6 +879,9 @@ template <int NC>
 struct JacShape {
     static constexpr int NF = 16 + 2 * NC;
     static constexpr int MAXC = 2 * NC + 4;
+    // direct reduction: per direction, the NF property derivatives and the
+    // seeds of the block's variables in variable order (x, y, P, Sw, So, Sg)
+    static constexpr int RF = NF + 2 * NC + 4;
 };
 
 template <int NC, class T>
@@ -933,11 +936,11 @@ __global__ __launch_bounds__(256) void k_dprops(AsmArgs A, double* dp) {
 
 template <int NC>
 __device__ __forceinline__ void load_props_dual(const AsmArgs& A, const double* dp, int blk, int j,
-                                                PropsT<NC, DualV<1>>& p, int ncol = JacShape<NC>::MAXC) {
-    constexpr int NF = JacShape<NC>::NF;
+                                                PropsT<NC, DualV<1>>& p, int ncol = JacShape<NC>::MAXC,
+                                                int nfield = JacShape<NC>::NF) {
     const int64_t s = 2 * (int64_t)A.n;
     const double* q = A.props + blk;
-    const double* g = dp + (int64_t)blk * NF * ncol + j;
+    const double* g = dp + (int64_t)blk * nfield * ncol + j;
     auto F = [&](int f) {
         DualV<1> r;
         r.v = q[f * s];
@@ -1058,6 +1061,7 @@ __global__ __launch_bounds__(256) void k_jac_nb(AsmArgs A, const double* dp, con
     constexpr int MAXC = JacShape<NC>::MAXC;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= (int64_t)2 * A.n * 7 * MAXC) return;
+    // block-major: a block's seven slots write neighbouring CSR positions
     const int j = (int)(item % MAXC);
     const int slot = 1 + (int)((item / MAXC) % 7);
     const int blk = (int)(item / (7 * MAXC));
@@ -1460,49 +1464,47 @@ __global__ void k_reduce_expand(AsmArgs A, const double* S, const double* t, con
 //   k_red_local  one lane group per block: the columns eliminated to S, t;
 //   k_red_dprops one lane per (block, direction q <= NP): the property
 //                derivatives along v_q / w, combined from the per-unknown
-//                ones (the property sweep is linear in the seed);
+//                ones (the property sweep is linear in the seed), and the
+//                direction's seeds of the block's variables;
 //   k_red_own / k_red_nb  the balance rows along direction q of the own
 //                block / of dependency slot 1..7, written to the reduced
 //                block's column q, or (q = NP) to a per-slot partial sum;
 //   k_red_rhs    one lane per block: -F_m plus the slot partials.
 
-// the block's unknowns seeded along reduced direction q (see above); nl is
-// the number of secondaries in play (0 for a zero-porosity block)
+// unknown column of variable m of a block in variable order (x_0..x_{nc-1},
+// y_0..y_{nc-1}, P, Sw, So, Sg), or -1 where the block has no such unknown
+// (InitGuess2's order, PetscAUX.cpp:892-957)
+__device__ __forceinline__ int var_col(int lab, int nc, int m) {
+    if (lab == 0) return m;
+    if (m < nc) return lab == 1 ? m : -1;
+    if (m < 2 * nc) return lab == -1 ? m - nc : -1;
+    const int sidx = m - 2 * nc;   // 0 P, 1 Sw, 2 So, 3 Sg
+    if (sidx <= 1) return nc + sidx;
+    if (sidx == 2) return lab == 1 ? nc + 2 : -1;
+    return lab == -1 ? nc + 2 : -1;
+}
+
+// the block's variables seeded along reduced direction q from the seeds
+// k_red_dprops tabulated (g: the block's column q, stride ncol)
 template <int NC>
-__device__ __forceinline__ void seed_dir(const VarsT<NC, double>& v, const double* Sb, const double* tb, int nl,
-                                         int q, double pcoeff, VarsT<NC, DualV<1>>& out) {
-    constexpr int NP = NC + 1;
+__device__ __forceinline__ void seed_tab(const VarsT<NC, double>& v, const double* g, int ncol, double pcoeff,
+                                         VarsT<NC, DualV<1>>& out) {
+    constexpr int NF = JacShape<NC>::NF;
     out.lab = v.lab;
-    out.Po = DualV<1>(v.Po);
+    auto D = [&](double val, int m, double sc) {
+        DualV<1> r;
+        r.v = val;
+        r.d[0] = g[(NF + m) * ncol] * sc;
+        return r;
+    };
 #pragma unroll
     for (int i = 0; i < NC; i++) {
-        out.x[i] = DualV<1>(v.x[i]);
-        out.y[i] = DualV<1>(v.y[i]);
+        out.x[i] = D(v.x[i], i, 1.0);
+        out.y[i] = D(v.y[i], NC + i, 1.0);
     }
+    out.Po = D(v.Po, 2 * NC, pcoeff);
 #pragma unroll
-    for (int j = 0; j < 3; j++) out.S[j] = DualV<1>(v.S[j]);
-    int col = 0;
-    auto mark = [&](DualV<1>& t, double sc) {
-        const int role = red_role(v.lab, NC, col);
-        double dv = 0.0;
-        if (role >= 0) {
-            dv = (role == q) ? 1.0 : 0.0;
-        } else {
-            const int si = -1 - role;
-            if (si < nl) dv = q < NP ? -Sb[si * NP + q] : tb[si];
-        }
-        t.d[0] = dv * sc;
-        col++;
-    };
-    // unknown order of seed_vars (InitGuess2, PetscAUX.cpp:892-957)
-    if (v.lab != -1)
-        for (int i = 0; i < NC; i++) mark(out.x[i], 1.0);
-    if (v.lab != 1)
-        for (int i = 0; i < NC; i++) mark(out.y[i], 1.0);
-    mark(out.Po, pcoeff);
-    mark(out.S[0], 1.0);
-    if (v.lab != -1) mark(out.S[1], 1.0);
-    if (v.lab != 1) mark(out.S[2], 1.0);
+    for (int j = 0; j < 3; j++) out.S[j] = D(v.S[j], 2 * NC + 1 + j, 1.0);
 }
 
 // one lane per (block, unknown j): column j of the block's local rows
@@ -1608,7 +1610,7 @@ __global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* dp,
     }
     const int pc = q < NP ? red_prim_col(lab, NC, q) : -1;
     const double* g = dp + (int64_t)blk * NF * MAXC;
-    double* o = dred + (int64_t)blk * NF * (NP + 1) + q;
+    double* o = dred + (int64_t)blk * JacShape<NC>::RF * (NP + 1) + q;
     for (int f = 0; f < NF; f++) {
         const double* gf = g + f * MAXC;
         double v = pc >= 0 ? gf[pc] : 0.0;
@@ -1616,6 +1618,22 @@ __global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* dp,
         for (int i = 0; i < NC + 3; i++)
             if (i < nl) v += wgt[i] * gf[sc[i]];
         o[f * (NP + 1)] = v;
+    }
+    // the variables' seeds: 1 on primary q, -S[s][q] (or t[s] for q = NP) on
+    // secondary s, 0 elsewhere and for variables the block does not have
+    for (int m = 0; m < 2 * NC + 4; m++) {
+        const int jc = var_col(lab, NC, m);
+        double dv = 0.0;
+        if (jc >= 0) {
+            const int role = red_role(lab, NC, jc);
+            if (role >= 0) {
+                dv = role == q ? 1.0 : 0.0;
+            } else {
+                const int si = -1 - role;
+                if (si < nl) dv = q < NP ? -Sb[si * NP + q] : tb[si];
+            }
+        }
+        o[(NF + m) * (NP + 1)] = dv;
     }
 }
 
@@ -1653,9 +1671,9 @@ __global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, 
     VarsT<NC, double> v;
     load_vars<NC>(A, k, c, v);
     VarsT<NC, D1> sv;
-    seed_dir<NC>(v, S + (int64_t)blk * NLM * NP, t + (int64_t)blk * NLM, nloc(v.lab, NC), q, A.pcoeff, sv);
+    seed_tab<NC>(v, dred + (int64_t)blk * JacShape<NC>::RF * (NP + 1) + q, NP + 1, A.pcoeff, sv);
     PropsT<NC, D1> sp;
-    load_props_dual<NC>(A, dred, blk, q, sp, NP + 1);
+    load_props_dual<NC>(A, dred, blk, q, sp, NP + 1, JacShape<NC>::RF);
     RowAcc<NC, D1> R;
     own_piece<NC, D1>(A, k, c, G, sv, sp, R);
     for (int f = 0; f < 6; f++) {
@@ -1696,9 +1714,12 @@ __global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, c
     constexpr int NLM = NC + 3, NP = NC + 1;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= (int64_t)2 * A.n * 7 * (NP + 1)) return;
+    // slot-major: the lanes of a wave share one slot, hence one code path
+    // (face or transfer) and one neighbour direction
     const int q = (int)(item % (NP + 1));
-    const int slot = 1 + (int)((item / (NP + 1)) % 7);
-    const int blk = (int)(item / (7 * (NP + 1)));
+    const int64_t rest = item / (NP + 1);
+    const int blk = (int)(rest % (2 * (int64_t)A.n));
+    const int slot = 1 + (int)(rest / (2 * (int64_t)A.n));
     const int n = A.n, k = blk / n, c = blk % n;
     if (A.phi[k * n + c] < 1e-10) return;
     const Geo G = geom(A, c);
@@ -1711,11 +1732,10 @@ __global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, c
     load_props<NC>(A, k, c, p);
     VarsT<NC, double> vd;
     load_vars<NC>(A, dk, dc, vd);
-    const int dnl = A.phi[dk * n + dc] < 1e-10 ? 0 : nloc(vd.lab, NC);
     VarsT<NC, D1> sv;
-    seed_dir<NC>(vd, S + (int64_t)d * NLM * NP, t + (int64_t)d * NLM, dnl, q, A.pcoeff, sv);
+    seed_tab<NC>(vd, dred + (int64_t)d * JacShape<NC>::RF * (NP + 1) + q, NP + 1, A.pcoeff, sv);
     PropsT<NC, D1> sp;
-    load_props_dual<NC>(A, dred, d, q, sp, NP + 1);
+    load_props_dual<NC>(A, dred, d, q, sp, NP + 1, JacShape<NC>::RF);
     RowAcc<NC, D1> R;
     if (slot == 7) {
         if (k == 0) transfer_piece<NC, double, D1>(A, 0, G, c, v, p, sv, sp, R);
@@ -1734,87 +1754,6 @@ __global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, c
     if (q == 0) bcols[bz] = d;
 #pragma unroll
     for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
-}
-
-// properties as duals from their values and a derivative column g[f * ncol]
-template <int NC>
-__device__ __forceinline__ void dual_props(const PropsT<NC, double>& pv, const double* g, int ncol,
-                                           PropsT<NC, DualV<1>>& p) {
-    auto F = [&](double v, int f) {
-        DualV<1> r;
-        r.v = v;
-        r.d[0] = g[f * ncol];
-        return r;
-    };
-    p.porpor = F(pv.porpor, 0); p.Pw = F(pv.Pw, 1); p.Po = F(pv.Po, 2); p.Pg = F(pv.Pg, 3);
-    p.bift = F(pv.bift, 4);
-    p.ro[0] = F(pv.ro[0], 5); p.ro[1] = F(pv.ro[1], 6); p.mu[0] = F(pv.mu[0], 7); p.mu[1] = F(pv.mu[1], 8);
-    p.gam[0] = F(pv.gam[0], 9); p.gam[1] = F(pv.gam[1], 10);
-    p.kr[0] = F(pv.kr[0], 11); p.kr[1] = F(pv.kr[1], 12); p.kr[2] = F(pv.kr[2], 13);
-    p.pcw = F(pv.pcw, 14); p.pcg = F(pv.pcg, 15);
-#pragma unroll
-    for (int i = 0; i < NC; i++) {
-        p.D[0][i] = F(pv.D[0][i], 16 + i);
-        p.D[1][i] = F(pv.D[1][i], 16 + NC + i);
-    }
-}
-
-// k_red_nb with one lane per (row block, slot) looping over the directions:
-// the dependency's values are loaded once, so the value half of every dual
-// operation is loop-invariant
-template <int NC>
-__global__ __launch_bounds__(256) void k_red_nb_loop(AsmArgs A, const double* dred, const double* S, const double* t,
-                                                     const int64_t* browptr, int32_t* bcols, double* bvals,
-                                                     double* rpart) {
-    using D1 = DualV<1>;
-    constexpr int NLM = NC + 3, NP = NC + 1, NF = JacShape<NC>::NF;
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= (int64_t)2 * A.n * 7) return;
-    const int slot = 1 + (int)(item % 7);
-    const int blk = (int)(item / 7);
-    const int n = A.n, k = blk / n, c = blk % n;
-    if (A.phi[k * n + c] < 1e-10) return;
-    const Geo G = geom(A, c);
-    const int d = dep_block(A, k, G, c, slot);
-    if (d < 0) return;
-    const int dk = d / n, dc = d % n;
-    VarsT<NC, double> v;
-    load_vars<NC>(A, k, c, v);
-    PropsT<NC, double> p;
-    load_props<NC>(A, k, c, p);
-    VarsT<NC, double> vd;
-    load_vars<NC>(A, dk, dc, vd);
-    PropsT<NC, double> pd;
-    load_props<NC>(A, dk, dc, pd);
-    const int dnl = A.phi[dk * n + dc] < 1e-10 ? 0 : nloc(vd.lab, NC);
-    const double* Sd = S + (int64_t)d * NLM * NP;
-    const double* td = t + (int64_t)d * NLM;
-    const double* gd = dred + (int64_t)d * NF * (NP + 1);
-    const int64_t bz = browptr[blk] + dep_rank(A, k, G, c, d);
-    bcols[bz] = d;
-    const FaceGeo FG = face_geo(A, G, slot - 1);
-#pragma unroll 1
-    for (int q = 0; q <= NP; q++) {
-        VarsT<NC, D1> sv;
-        seed_dir<NC>(vd, Sd, td, dnl, q, A.pcoeff, sv);
-        PropsT<NC, D1> sp;
-        dual_props<NC>(pd, gd + q, NP + 1, sp);
-        RowAcc<NC, D1> R;
-        if (slot == 7) {
-            if (k == 0) transfer_piece<NC, double, D1>(A, 0, G, c, v, p, sv, sp, R);
-            else transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, v, p, R);
-        } else {
-            face_piece<NC, double, D1>(A, k, G, FG, slot - 1, c, dc, v, p, sv, sp, R);
-        }
-        if (q == NP) {
-            double* o = rpart + ((int64_t)blk * 8 + slot) * NP;
-#pragma unroll
-            for (int i = 0; i < NP; i++) o[i] = R.bal[i].d[0];
-        } else {
-#pragma unroll
-            for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
-        }
-    }
 }
 
 template <int NC>
@@ -2028,7 +1967,8 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     const int64_t nb = 2 * (int64_t)a->n;
     if (!a->dprops) OSR_HIP(hipMalloc((void**)&a->dprops, (size_t)nb * NF * MAXC * sizeof(double)));
     if (!a->wloc) OSR_HIP(hipMalloc((void**)&a->wloc, (size_t)nb * MAXC * (NC + 3) * sizeof(double)));
-    if (!a->dred) OSR_HIP(hipMalloc((void**)&a->dred, (size_t)nb * NF * (NP + 1) * sizeof(double)));
+    if (!a->dred)
+        OSR_HIP(hipMalloc((void**)&a->dred, (size_t)nb * osr::JacShape<NC>::RF * (NP + 1) * sizeof(double)));
     if (!a->rpart) OSR_HIP(hipMalloc((void**)&a->rpart, (size_t)nb * 8 * NP * sizeof(double)));
     const unsigned g1 = (unsigned)((nb + 255) / 256);
     const unsigned gc = (unsigned)((nb * MAXC + 255) / 256);
@@ -2215,7 +2155,8 @@ int osr_asm_destroy(osr_asm* a) {
 static size_t ws_layout(const osr_asm* a, size_t off[4]) {
     const size_t nb = 2 * (size_t)a->n, nc = (size_t)a->nc;
     const size_t NF = 16 + 2 * nc, MAXC = 2 * nc + 4, NP = nc + 1;
-    const size_t sz[4] = {nb * NF * MAXC * 8, nb * MAXC * (nc + 3) * 8, nb * NF * (NP + 1) * 8, nb * 8 * NP * 8};
+    const size_t sz[4] = {nb * NF * MAXC * 8, nb * MAXC * (nc + 3) * 8, nb * (NF + MAXC) * (NP + 1) * 8,
+                          nb * 8 * NP * 8};
     size_t at = 0;
     for (int i = 0; i < 4; i++) {
         off[i] = at;

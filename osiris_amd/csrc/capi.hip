@@ -87,6 +87,7 @@ int osr_ctx_destroy(osr_ctx* ctx) {
     if (ctx->d_fluid) hipFree(ctx->d_fluid);
     if (ctx->d_counters) hipFree(ctx->d_counters);
     if (ctx->d_queue) hipFree(ctx->d_queue);
+    if (ctx->d_plist) hipFree(ctx->d_plist);
     if (ctx->d_park) hipFree(ctx->d_park);
     hipSetDevice(dev);
     delete ctx;

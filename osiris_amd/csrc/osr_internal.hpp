@@ -27,6 +27,9 @@ struct osr_ctx {
     int park_after = 40;            // persistent flash: passes before a cell moves to the lane-group kernel
     void* d_park = nullptr;
     size_t park_bytes = 0;
+    // Saturation-pressure work list: [count (8 bytes) | cell indices int32[n]]
+    void* d_plist = nullptr;
+    size_t plist_bytes = 0;
     int mixed_warm_iters = 3;
     float mixed_tol32 = 1e-4f;
     int mixed_max32 = 150;

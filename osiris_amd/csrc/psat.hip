@@ -10,6 +10,8 @@
 // mean relative step <= 1e-8.  The (nc+1)^2 Newton matrix of each lane lives
 // in LDS, laid out [element][lane] so the 64 lanes of a wave hit 64 distinct
 // banks; LU with partial pivoting (the dgesv arithmetic) runs in place.
+#include <algorithm>
+
 #include "osr_internal.hpp"
 
 namespace osr {
@@ -30,6 +32,8 @@ struct PsatArgs {
     const double* y;
     double* psat;
     double* xm;
+    const int32_t* list;                  // cells with a test to run (kind +-1)
+    const unsigned long long* count;      // their number
 };
 
 // Phase state with the sensitivities the Newton stage needs.
@@ -89,22 +93,56 @@ __device__ __forceinline__ void dlnphi_dp(const FluidConsts<double>& C, double T
     }
 }
 
+// Cells without a test (kind 0) get psat = -1 and xm = 0 here; the others
+// are appended to the work list (one atomic per wave, wave-aggregated), so
+// the saturation kernel only schedules cells that run a test.
+__global__ void k_psat_select(int64_t n, int nc, const int8_t* kind, double* psat, double* xm, int32_t* list,
+                              unsigned long long* count) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false;
+    if (c < n) {
+        const int kd = kind[c];
+        act = kd == 1 || kd == -1;
+        if (!act) {
+            psat[c] = -1;
+            if (xm)
+                for (int i = 0; i < nc; i++) xm[(int64_t)i * n + c] = 0;
+        }
+    }
+    const unsigned long long m = __ballot(act);
+    if (m == 0) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+    base = (unsigned long long)__shfl((long long)base, leader, 64);
+    if (act) list[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)c;
+}
+
+__global__ void k_psat_zero(unsigned long long* count) { *count = 0; }
+
+template <int NC>
+__device__ __forceinline__ void psat_cell(const FluidDev* __restrict__ F, const PsatArgs& a, int64_t c, int lane,
+                                          double* sJ);
+
+// One lane per listed cell (grid-stride over the work list).  The Newton
+// matrix of each lane lives in LDS, laid out [element][lane].
 template <int NC>
 __global__ __launch_bounds__(64) void k_psat(const FluidDev* __restrict__ F, PsatArgs a) {
-    constexpr int N1 = NC + 1;
-    extern __shared__ double sJ[];   // [N1*N1][64]
+    extern __shared__ double sJ[];   // [(NC+1)^2][64]
     const int lane = threadIdx.x;
-    const int64_t c = (int64_t)blockIdx.x * 64 + lane;
-    if (c >= a.n) return;
+    const int64_t cnt = (int64_t)*a.count;
+    for (int64_t i = (int64_t)blockIdx.x * 64 + lane; i < cnt; i += (int64_t)gridDim.x * 64)
+        psat_cell<NC>(F, a, (int64_t)a.list[i], lane, sJ);
+}
+
+template <int NC>
+__device__ __forceinline__ void psat_cell(const FluidDev* __restrict__ F, const PsatArgs& a, int64_t c, int lane,
+                                          double* sJ) {
+    constexpr int N1 = NC + 1;
     const FluidConsts<double>& C = F->d;
     const bool srk = F->eos == OSR_EOS_SRK;
     const int kind = a.kind[c];
-    if (kind != 1 && kind != -1) {
-        a.psat[c] = -1;
-        if (a.xm)
-            for (int i = 0; i < NC; i++) a.xm[(int64_t)i * a.n + c] = 0;
-        return;
-    }
     const bool bubble = kind == 1;
     const double T = a.T ? a.T[c] : a.T0;
     const double P0 = a.P[c];
@@ -343,7 +381,8 @@ __global__ __launch_bounds__(64) void k_psat(const FluidDev* __restrict__ F, Psa
 template <int NC>
 static int launch_psat_nc(osr_ctx* ctx, const PsatArgs& a, hipStream_t stream) {
     const size_t lds = (size_t)(NC + 1) * (NC + 1) * 64 * sizeof(double);
-    const int64_t grid = (a.n + 63) / 64;
+    const int64_t per_cu = 4;   // more workgroups than fit at once at nc >= 8 (LDS-bound), for balance
+    const int64_t grid = std::min<int64_t>((a.n + 63) / 64, (int64_t)(ctx->num_cu > 0 ? ctx->num_cu : 256) * per_cu);
     if (lds > 65536)
         OSR_HIP(hipFuncSetAttribute((const void*)k_psat<NC>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -358,7 +397,20 @@ int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const doubl
     if (n == 0) return OSR_OK;
     if (!P || !kind || !x || !y || !psat) return OSR_E_ARG;
     if (!T && !(T0 > 0)) return OSR_E_ARG;
-    PsatArgs a{n, T, T0, P, kind, x, y, psat, xm};
+    const size_t need = 256 + (size_t)n * sizeof(int32_t);
+    if (need > ctx->plist_bytes) {
+        if (ctx->d_plist) OSR_HIP(hipFree(ctx->d_plist));
+        ctx->d_plist = nullptr;
+        ctx->plist_bytes = 0;
+        OSR_HIP(hipMalloc(&ctx->d_plist, need));
+        ctx->plist_bytes = need;
+    }
+    unsigned long long* count = (unsigned long long*)ctx->d_plist;
+    int32_t* list = (int32_t*)((char*)ctx->d_plist + 256);
+    hipLaunchKernelGGL(k_psat_zero, dim3(1), dim3(1), 0, stream, count);
+    hipLaunchKernelGGL(k_psat_select, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, ctx->nc, kind,
+                       psat, xm, list, count);
+    PsatArgs a{n, T, T0, P, kind, x, y, psat, xm, list, count};
     switch (ctx->nc) {
         case 2: return launch_psat_nc<2>(ctx, a, stream);
         case 3: return launch_psat_nc<3>(ctx, a, stream);
