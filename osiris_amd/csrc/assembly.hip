@@ -1466,9 +1466,10 @@ __global__ void k_reduce_expand(AsmArgs A, const double* S, const double* t, con
 //                derivatives along v_q / w, combined from the per-unknown
 //                ones (the property sweep is linear in the seed), and the
 //                direction's seeds of the block's variables;
-//   k_red_own / k_red_nb  the balance rows along direction q of the own
-//                block / of dependency slot 1..7, written to the reduced
-//                block's column q, or (q = NP) to a per-slot partial sum;
+//   k_red_own / k_red_nb / k_red_tr  the balance rows along direction q of
+//                the own block / of a face neighbour / of the other
+//                continuum's block, written to the reduced block's column q,
+//                or (q = NP) to a per-slot partial sum;
 //   k_red_rhs    one lane per block: -F_m plus the slot partials.
 
 // unknown column of variable m of a block in variable order (x_0..x_{nc-1},
@@ -1706,51 +1707,87 @@ __global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, 
     for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
 }
 
+// Faces, column-block-major: one lane per (column block d, face f',
+// direction q).  The row block b is d's neighbour across f' and d sits
+// across the opposite face of b; the lanes of one d run together, so d's
+// derivative table is read once while b's values are gathered.
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, const double* S, const double* t,
-                                                const int64_t* browptr, int32_t* bcols, double* bvals,
-                                                double* rpart) {
+__global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, const int64_t* browptr,
+                                                int32_t* bcols, double* bvals, double* rpart) {
     using D1 = DualV<1>;
-    constexpr int NLM = NC + 3, NP = NC + 1;
+    constexpr int NP = NC + 1;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= (int64_t)2 * A.n * 7 * (NP + 1)) return;
-    // slot-major: the lanes of a wave share one slot, hence one code path
-    // (face or transfer) and one neighbour direction
+    if (item >= (int64_t)2 * A.n * 6 * (NP + 1)) return;
     const int q = (int)(item % (NP + 1));
     const int64_t rest = item / (NP + 1);
-    const int blk = (int)(rest % (2 * (int64_t)A.n));
-    const int slot = 1 + (int)(rest / (2 * (int64_t)A.n));
-    const int n = A.n, k = blk / n, c = blk % n;
-    if (A.phi[k * n + c] < 1e-10) return;
-    const Geo G = geom(A, c);
-    const int d = dep_block(A, k, G, c, slot);
-    if (d < 0) return;
-    const int dk = d / n, dc = d % n;
+    const int fp = (int)(rest % 6);
+    const int d = (int)(rest / 6);
+    const int n = A.n, k = d / n, cd = d % n;
+    const int cb = nb_of(A, geom(A, cd), fp);
+    if (cb < 0 || A.phi[k * n + cb] < 1e-10) return;   // no neighbour / identity row block
+    const int b = k * n + cb;
+    const int f = fp ^ 1;                                // d as seen from b
+    const Geo G = geom(A, cb);
     VarsT<NC, double> v;
-    load_vars<NC>(A, k, c, v);
+    load_vars<NC>(A, k, cb, v);
     PropsT<NC, double> p;
-    load_props<NC>(A, k, c, p);
+    load_props<NC>(A, k, cb, p);
     VarsT<NC, double> vd;
-    load_vars<NC>(A, dk, dc, vd);
+    load_vars<NC>(A, k, cd, vd);
     VarsT<NC, D1> sv;
     seed_tab<NC>(vd, dred + (int64_t)d * JacShape<NC>::RF * (NP + 1) + q, NP + 1, A.pcoeff, sv);
     PropsT<NC, D1> sp;
     load_props_dual<NC>(A, dred, d, q, sp, NP + 1, JacShape<NC>::RF);
     RowAcc<NC, D1> R;
-    if (slot == 7) {
-        if (k == 0) transfer_piece<NC, double, D1>(A, 0, G, c, v, p, sv, sp, R);
-        else transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, v, p, R);
-    } else {
-        const FaceGeo FG = face_geo(A, G, slot - 1);
-        face_piece<NC, double, D1>(A, k, G, FG, slot - 1, c, dc, v, p, sv, sp, R);
-    }
+    const FaceGeo FG = face_geo(A, G, f);
+    face_piece<NC, double, D1>(A, k, G, FG, f, cb, cd, v, p, sv, sp, R);
     if (q == NP) {
         const int64_t nb = 2 * (int64_t)n;
 #pragma unroll
-        for (int i = 0; i < NP; i++) rpart[((int64_t)slot * NP + i) * nb + blk] = R.bal[i].d[0];
+        for (int i = 0; i < NP; i++) rpart[((int64_t)(f + 1) * NP + i) * nb + b] = R.bal[i].d[0];
         return;
     }
-    const int64_t bz = browptr[blk] + dep_rank(A, k, G, c, d);
+    const int64_t bz = browptr[b] + dep_rank(A, k, G, cb, d);
+    if (q == 0) bcols[bz] = d;
+#pragma unroll
+    for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
+}
+
+// Transfer, column-block-major: one lane per (column block d, direction q);
+// the row block is the other continuum's block of the same cell.
+template <int NC>
+__global__ __launch_bounds__(256) void k_red_tr(AsmArgs A, const double* dred, const int64_t* browptr,
+                                                int32_t* bcols, double* bvals, double* rpart) {
+    using D1 = DualV<1>;
+    constexpr int NP = NC + 1;
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (int64_t)2 * A.n * (NP + 1)) return;
+    const int q = (int)(item % (NP + 1));
+    const int d = (int)(item / (NP + 1));
+    const int n = A.n, kd = d / n, c = d % n, kb = 1 - kd;
+    if (A.phi[kb * n + c] < 1e-10) return;
+    const int b = kb * n + c;
+    const Geo G = geom(A, c);
+    VarsT<NC, double> v;
+    load_vars<NC>(A, kb, c, v);
+    PropsT<NC, double> p;
+    load_props<NC>(A, kb, c, p);
+    VarsT<NC, double> vd;
+    load_vars<NC>(A, kd, c, vd);
+    VarsT<NC, D1> sv;
+    seed_tab<NC>(vd, dred + (int64_t)d * JacShape<NC>::RF * (NP + 1) + q, NP + 1, A.pcoeff, sv);
+    PropsT<NC, D1> sp;
+    load_props_dual<NC>(A, dred, d, q, sp, NP + 1, JacShape<NC>::RF);
+    RowAcc<NC, D1> R;
+    if (kb == 0) transfer_piece<NC, double, D1>(A, 0, G, c, v, p, sv, sp, R);
+    else transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, v, p, R);
+    if (q == NP) {
+        const int64_t nb = 2 * (int64_t)n;
+#pragma unroll
+        for (int i = 0; i < NP; i++) rpart[((int64_t)7 * NP + i) * nb + b] = R.bal[i].d[0];
+        return;
+    }
+    const int64_t bz = browptr[b] + dep_rank(A, kb, G, c, d);
     if (q == 0) bcols[bz] = d;
 #pragma unroll
     for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
@@ -1973,7 +2010,7 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     const unsigned g1 = (unsigned)((nb + 255) / 256);
     const unsigned gc = (unsigned)((nb * MAXC + 255) / 256);
     const unsigned gq = (unsigned)((nb * (NP + 1) + 255) / 256);
-    const unsigned gn = (unsigned)((nb * 7 * (NP + 1) + 255) / 256);
+    const unsigned gn = (unsigned)((nb * 6 * (NP + 1) + 255) / 256);
     hipLaunchKernelGGL(osr::k_dprops<NC>, dim3(gc), dim3(256), 0, st, A, a->dprops);
     hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3(gc), dim3(256), 0, st, A, (const double*)a->dprops, a->wloc);
     hipLaunchKernelGGL((osr::k_red_local<NC, false>), dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256),
@@ -1982,8 +2019,10 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
                        (const double*)S, (const double*)t, a->dred);
     hipLaunchKernelGGL(osr::k_red_own<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, (const double*)S,
                        (const double*)t, browptr, bcols, bvals, a->rpart);
-    hipLaunchKernelGGL(osr::k_red_nb<NC>, dim3(gn), dim3(256), 0, st, A, (const double*)a->dred, (const double*)S,
-                       (const double*)t, browptr, bcols, bvals, a->rpart);
+    hipLaunchKernelGGL(osr::k_red_nb<NC>, dim3(gn), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
+                       bvals, a->rpart);
+    hipLaunchKernelGGL(osr::k_red_tr<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
+                       bvals, a->rpart);
     hipLaunchKernelGGL(osr::k_red_rhs<NC>, dim3(g1), dim3(256), 0, st, A, F, (const double*)a->rpart, brhs);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
