@@ -183,8 +183,9 @@ int osr_asm_destroy(osr_asm *a);
 
 /* Jacobian workspace (no counterpart in the reference, whose PETSc Mat is the
  * only buffer).  The Jacobian and reduced-system calls need scratch of
- * *bytes ((16+2nc)(2nc+4) + (2nc+4)(nc+3) + (16+4nc+4)(nc+2) + 8(nc+1)
- * doubles per block, 15.5 KB at nc = 10); by default each
+ * *bytes (per block, the larger of (16+2nc)(2nc+4) doubles for the CSR
+ * Jacobian and (2nc+4)(nc+3) + (20+4nc)(nc+2) + 8(nc+1) for the direct
+ * reduced system: 8.8 KB at nc = 10); by default each
  * osr_asm allocates its own on first use.  osr_asm_set_workspace hands it
  * a caller-owned device buffer instead (256-byte aligned, at least *bytes;
  * NULL returns to a library-owned one), so assemblers whose calls are

@@ -25,6 +25,8 @@
 // bitwise reproducible.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "dual_dev.hpp"
 #include "osr_internal.hpp"
 
@@ -310,6 +312,39 @@ __device__ __forceinline__ void propsT(const AsmArgs& A, int k, int c, int lab, 
     }
     p.gam[0] = G_ACCG * p.ro[0] * ml;
     p.gam[1] = G_ACCG * p.ro[1] * mg;
+}
+
+// The part of the property sweep the block-local rows read: Po, Pg and the
+// two phase densities (and the pore-volume factor), by the same expressions
+// as propsT.
+template <int NC, class T>
+__device__ __forceinline__ void propsT_loc(const AsmArgs& A, int k, int c, int lab, const T& Po, const T* x,
+                                           const T* y, const T* S, double rpo, double rpg, PropsT<NC, T>& p) {
+    const AsmConsts& K = *A.K;
+    const double phi = A.phi[k * A.n + c];
+    const T dP = Po - K.refP;
+    p.Po = Po;
+    p.porpor = phi * (1.0 + K.cpor * dP + K.dcpor * dP * dP);
+    if (k == 0) {
+        if (lab == 0) {
+            T ps = 0.0;
+#pragma unroll
+            for (int i = 0; i < NC; i++) ps += K.par[i] * (x[i] * rpo - y[i] * rpg);
+            p.bift = fpow(ps * 1e-6, 3.88);
+        } else {
+            p.bift = T(IFTBASECASE);
+        }
+        T krg, krog, pcg;
+        table<T>(K.sgt, K.nsgt, S[2], krg, krog, pcg);
+        if (p.bift < MISCIBLEIFT) p.Pg = Po + pcg * p.bift / IFTBASECASE;
+        else p.Pg = Po + pcg;
+    } else {
+        p.bift = T(IFTBASECASE);
+        p.Pg = Po;
+    }
+    const double rt = RGAS * K.T;
+    p.ro[0] = p.Po / (zphase<NC, T>(K, x, p.Po, false) * rt);
+    p.ro[1] = p.Pg / (zphase<NC, T>(K, y, p.Pg, true) * rt);
 }
 
 // state of one block as T
@@ -663,6 +698,10 @@ __device__ __forceinline__ void transfer_piece(const AsmArgs& A, int k, const Ge
 // block and the constraint rows: everything that depends on the block's own
 // unknowns only.
 template <int NC, class T>
+__device__ __forceinline__ void loc_piece(const AsmArgs& A, const VarsT<NC, T>& v, const PropsT<NC, T>& p,
+                                          RowAcc<NC, T>& R);
+
+template <int NC, class T>
 __device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const Geo& G, const VarsT<NC, T>& v,
                                           const PropsT<NC, T>& p, RowAcc<NC, T>& R) {
     const AsmConsts& K = *A.K;
@@ -682,6 +721,15 @@ __device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const 
         const double pr = pre ? pre[(int64_t)NC * n + c] : dval(acc);
         R.bal[NC] += f * (acc - pr);
     }
+    loc_piece<NC, T>(A, v, p, R);
+}
+
+// The block-local rows: fugacity equalities of a two-phase block and the
+// constraint rows (they read Po, Pg and the two phase densities only).
+template <int NC, class T>
+__device__ __forceinline__ void loc_piece(const AsmArgs& A, const VarsT<NC, T>& v, const PropsT<NC, T>& p,
+                                          RowAcc<NC, T>& R) {
+    const AsmConsts& K = *A.K;
     if (v.lab == 0) {
         T Cal, Cbl, Cag, Cbg, tl[NC], tg[NC];
         mixT<NC, T>(K, v.x, Cal, Cbl, tl);
@@ -1509,10 +1557,12 @@ __device__ __forceinline__ void seed_tab(const VarsT<NC, double>& v, const doubl
 }
 
 // one lane per (block, unknown j): column j of the block's local rows
-// (fugacity and constraint rows) by forward mode of its own piece, into Wl
-// laid out [block][column][local row] (a lane's column is contiguous)
+// (fugacity and constraint rows) by forward mode with unknown j seeded --
+// they read only Po, Pg and the phase densities, so only that part of the
+// property sweep runs (propsT_loc), with the values of the property table
+// -- into Wl laid out [block][column][local row]
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, const double* dp, double* Wl) {
+__global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, double* Wl) {
     using D1 = DualV<1>;
     constexpr int MAXC = JacShape<NC>::MAXC;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1523,15 +1573,25 @@ __global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, const double* dp, do
     const int lab = A.lab[k][c];
     const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
     if (j >= bs || A.phi[k * n + c] < 1e-10) return;
-    const Geo G = geom(A, c);
     VarsT<NC, double> v;
     load_vars<NC>(A, k, c, v);
     VarsT<NC, D1> sv;
     seed_vars<NC, 1>(v, j, A.pcoeff, sv);
+    double rpo = 0, rpg = 0;
+    if (k == 0 && A.rho_prev) {
+        rpo = A.rho_prev[c];
+        rpg = A.rho_prev[n + c];
+    }
     PropsT<NC, D1> sp;
-    load_props_dual<NC>(A, dp, blk, j, sp);
+    propsT_loc<NC, D1>(A, k, c, lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp);
+    const int64_t s2 = 2 * (int64_t)n;
+    const double* tab = A.props + blk;
+    sp.Po.v = tab[2 * s2];
+    sp.Pg.v = tab[3 * s2];
+    sp.ro[0].v = tab[5 * s2];
+    sp.ro[1].v = tab[6 * s2];
     RowAcc<NC, D1> R;
-    own_piece<NC, D1>(A, k, c, G, sv, sp, R);
+    loc_piece<NC, D1>(A, sv, sp, R);
     const int nl = nloc(lab, NC);
     double* w = Wl + ((int64_t)blk * MAXC + j) * (NC + 3);
 #pragma unroll
@@ -1589,10 +1649,13 @@ __global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, 
     group_write_st<NC>(col, lab, nl, j, bs, Sb, tb);
 }
 
+// one lane per (block, direction q <= NP): the property sweep in forward
+// mode along the reduced direction q of the block (v_q, or w for q = NP),
+// written with the direction's seeds of the block's variables
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* dp, const double* S, const double* t,
-                                                    double* dred) {
-    constexpr int NF = JacShape<NC>::NF, MAXC = JacShape<NC>::MAXC, NLM = NC + 3, NP = NC + 1;
+__global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* S, const double* t, double* dred) {
+    using D1 = DualV<1>;
+    constexpr int NF = JacShape<NC>::NF, NLM = NC + 3, NP = NC + 1;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= (int64_t)2 * A.n * (NP + 1)) return;
     const int q = (int)(item % (NP + 1));
@@ -1602,27 +1665,14 @@ __global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* dp,
     const int nl = A.phi[k * n + c] < 1e-10 ? 0 : nloc(lab, NC);
     const double* Sb = S + (int64_t)blk * NLM * NP;
     const double* tb = t + (int64_t)blk * NLM;
-    double wgt[NC + 3];
-    int sc[NC + 3];
-#pragma unroll
-    for (int i = 0; i < NC + 3; i++) {
-        wgt[i] = i < nl ? (q < NP ? -Sb[i * NP + q] : tb[i]) : 0.0;
-        sc[i] = i < nl ? red_sec_col(lab, NC, i) : 0;
-    }
-    const int pc = q < NP ? red_prim_col(lab, NC, q) : -1;
-    const double* g = dp + (int64_t)blk * NF * MAXC;
     double* o = dred + (int64_t)blk * JacShape<NC>::RF * (NP + 1) + q;
-    for (int f = 0; f < NF; f++) {
-        const double* gf = g + f * MAXC;
-        double v = pc >= 0 ? gf[pc] : 0.0;
-#pragma unroll
-        for (int i = 0; i < NC + 3; i++)
-            if (i < nl) v += wgt[i] * gf[sc[i]];
-        o[f * (NP + 1)] = v;
-    }
     // the variables' seeds: 1 on primary q, -S[s][q] (or t[s] for q = NP) on
     // secondary s, 0 elsewhere and for variables the block does not have
-    for (int m = 0; m < 2 * NC + 4; m++) {
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    VarsT<NC, D1> sv;
+    sv.lab = v.lab;
+    auto seed = [&](double val, int m, double sc, D1& out) -> double {
         const int jc = var_col(lab, NC, m);
         double dv = 0.0;
         if (jc >= 0) {
@@ -1634,8 +1684,35 @@ __global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* dp,
                 if (si < nl) dv = q < NP ? -Sb[si * NP + q] : tb[si];
             }
         }
-        o[(NF + m) * (NP + 1)] = dv;
+        out.v = val;
+        out.d[0] = dv * sc;
+        return dv;
+    };
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        seed(v.x[i], i, 1.0, sv.x[i]);
+        seed(v.y[i], NC + i, 1.0, sv.y[i]);
     }
+    const double dvP = seed(v.Po, 2 * NC, A.pcoeff, sv.Po);
+#pragma unroll
+    for (int j = 0; j < 3; j++) seed(v.S[j], 2 * NC + 1 + j, 1.0, sv.S[j]);
+    double rpo = 0, rpg = 0;
+    if (k == 0 && A.rho_prev) {
+        rpo = A.rho_prev[c];
+        rpg = A.rho_prev[n + c];
+    }
+    PropsT<NC, D1> sp;
+    propsT<NC, D1>(A, k, c, sv.lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp);
+#pragma unroll
+    for (int f = 0; f < NF; f++) o[f * (NP + 1)] = prop_field<NC, D1>(sp, f).d[0];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        o[(NF + i) * (NP + 1)] = sv.x[i].d[0];
+        o[(NF + NC + i) * (NP + 1)] = sv.y[i].d[0];
+    }
+    o[(NF + 2 * NC) * (NP + 1)] = dvP;
+#pragma unroll
+    for (int j = 0; j < 3; j++) o[(NF + 2 * NC + 1 + j) * (NP + 1)] = sv.S[j].d[0];
 }
 
 // rank of dependency d among the row block's existing dependencies in
@@ -2002,7 +2079,6 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     constexpr int NF = osr::JacShape<NC>::NF, MAXC = osr::JacShape<NC>::MAXC;
     constexpr int NP = NC + 1;
     const int64_t nb = 2 * (int64_t)a->n;
-    if (!a->dprops) OSR_HIP(hipMalloc((void**)&a->dprops, (size_t)nb * NF * MAXC * sizeof(double)));
     if (!a->wloc) OSR_HIP(hipMalloc((void**)&a->wloc, (size_t)nb * MAXC * (NC + 3) * sizeof(double)));
     if (!a->dred)
         OSR_HIP(hipMalloc((void**)&a->dred, (size_t)nb * osr::JacShape<NC>::RF * (NP + 1) * sizeof(double)));
@@ -2011,12 +2087,11 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     const unsigned gc = (unsigned)((nb * MAXC + 255) / 256);
     const unsigned gq = (unsigned)((nb * (NP + 1) + 255) / 256);
     const unsigned gn = (unsigned)((nb * 6 * (NP + 1) + 255) / 256);
-    hipLaunchKernelGGL(osr::k_dprops<NC>, dim3(gc), dim3(256), 0, st, A, a->dprops);
-    hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3(gc), dim3(256), 0, st, A, (const double*)a->dprops, a->wloc);
+    hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3(gc), dim3(256), 0, st, A, a->wloc);
     hipLaunchKernelGGL((osr::k_red_local<NC, false>), dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256),
                        0, st, A, (const double*)a->wloc, F, (const int64_t*)nullptr, (const double*)nullptr, S, t);
-    hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dprops,
-                       (const double*)S, (const double*)t, a->dred);
+    hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)S, (const double*)t,
+                       a->dred);
     hipLaunchKernelGGL(osr::k_red_own<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, (const double*)S,
                        (const double*)t, browptr, bcols, bvals, a->rpart);
     hipLaunchKernelGGL(osr::k_red_nb<NC>, dim3(gn), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
@@ -2190,18 +2265,21 @@ int osr_asm_destroy(osr_asm* a) {
     return OSR_OK;
 }
 
-// Jacobian workspace layout: [dprops | wloc | dred | rpart], 256-byte aligned
+// Jacobian workspace layout (256-byte aligned): the CSR Jacobian uses dprops
+// only, the direct reduced system wloc | dred | rpart only, so the two
+// overlay each other from offset 0 (calls on one stream run in order)
 static size_t ws_layout(const osr_asm* a, size_t off[4]) {
     const size_t nb = 2 * (size_t)a->n, nc = (size_t)a->nc;
     const size_t NF = 16 + 2 * nc, MAXC = 2 * nc + 4, NP = nc + 1;
-    const size_t sz[4] = {nb * NF * MAXC * 8, nb * MAXC * (nc + 3) * 8, nb * (NF + MAXC) * (NP + 1) * 8,
-                          nb * 8 * NP * 8};
-    size_t at = 0;
-    for (int i = 0; i < 4; i++) {
-        off[i] = at;
-        at += (sz[i] + 255) / 256 * 256;
-    }
-    return at;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t dprops = al(nb * NF * MAXC * 8);
+    const size_t wloc = al(nb * MAXC * (nc + 3) * 8), dred = al(nb * (NF + MAXC) * (NP + 1) * 8),
+                 rpart = al(nb * 8 * NP * 8);
+    off[0] = 0;
+    off[1] = 0;
+    off[2] = wloc;
+    off[3] = wloc + dred;
+    return std::max(dprops, wloc + dred + rpart);
 }
 
 int osr_asm_props_table(osr_asm* a, const double** props) {

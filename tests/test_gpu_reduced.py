@@ -5,9 +5,11 @@ Bars: the normwise backward error of the expanded update,
 |J du + F|_inf / (|J|_inf |du|_inf + |F|_inf), below 1e-12, and agreement
 with a sparse direct solve of the full system to 1e-7 of max |du|, for the
 reduction from the full CSR and for the one assembled straight from the
-state (no CSR).  The two reductions agree: S and t bitwise (both eliminate
-the same forward-mode local rows), blocks and right-hand sides within 1e-9
-of the largest entry of their reduced row (they differ by summation order:
+state (no CSR).  The two reductions agree: S and t within 1e-10 of each
+block's largest entry (the local rows come from the full property sweep in
+one and from its density/pressure part in the other, equal up to rounding,
+then one elimination), blocks and right-hand sides within 1e-9 of the
+largest entry of their reduced row (they differ by summation order:
 J_p - J_s S summed explicitly versus one forward-mode sweep along the
 combined direction).  (A
 componentwise bar is meaningless here: at the initial state many rows --
@@ -76,7 +78,9 @@ def _same_reduction(name, a, b, np1, nan_rows=False):
     for i, what in ((4, "S"), (5, "t")):
         x, y = a[i].cpu().numpy(), b[i].cpu().numpy()
         assert np.array_equal(np.isnan(x), np.isnan(y)), (name, what)
-        assert np.array_equal(np.nan_to_num(x), np.nan_to_num(y)), (name, what)
+        x, y = np.nan_to_num(x).reshape(len(x), -1), np.nan_to_num(y).reshape(len(y), -1)
+        scale = np.maximum(np.abs(x).max(axis=1), 1e-300)
+        assert (np.abs(x - y).max(axis=1) / scale).max() <= 1e-10, (name, what)
     bv_a, bv_b = a[2].cpu().numpy(), b[2].cpu().numpy()
     rh_a, rh_b = a[3].cpu().numpy(), b[3].cpu().numpy()
     if not nan_rows:
