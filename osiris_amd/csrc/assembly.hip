@@ -138,6 +138,21 @@ __device__ __forceinline__ DualV<W> zroot(const DualV<W>& a1, const DualV<W>& a2
     return r;
 }
 
+// Forward mode with the root's value already known (zk > 0, e.g. from the
+// property table): only the implicit-function derivative is formed.
+__device__ __forceinline__ double zroot_k(double a1, double a2, double a3, bool gas, double zk) {
+    return zroot(a1, a2, a3, gas);
+}
+template <int W>
+__device__ __forceinline__ DualV<W> zroot_k(const DualV<W>& a1, const DualV<W>& a2, const DualV<W>& a3, bool gas,
+                                            double zk) {
+    if (!(zk > 0)) return zroot(a1, a2, a3, gas);
+    const double inv = -1.0 / ((3.0 * zk + 2.0 * a1.v) * zk + a2.v);
+    DualV<W> r = (zk * zk * a1 + zk * a2 + a3) * inv;
+    r.v = zk;
+    return r;
+}
+
 template <int NC, class T>
 __device__ __forceinline__ void mixT(const AsmConsts& K, const T* x, T& Ca, T& Cb, T* t) {
     T u[NC];
@@ -157,7 +172,7 @@ __device__ __forceinline__ void mixT(const AsmConsts& K, const T* x, T& Ca, T& C
 }
 
 template <int NC, class T>
-__device__ __forceinline__ T zphase(const AsmConsts& K, const T* x, const T& P, bool gas) {
+__device__ __forceinline__ T zphase(const AsmConsts& K, const T* x, const T& P, bool gas, double zk = 0.0) {
     T Ca, Cb;
     mixT<NC, T>(K, x, Ca, Cb, (T*)nullptr);
     const double rt = RGAS * K.T;
@@ -166,7 +181,7 @@ __device__ __forceinline__ T zphase(const AsmConsts& K, const T* x, const T& P, 
     const bool srk = K.eos == OSR_EOS_SRK;
     const T U = srk ? B : 2.0 * B;
     const T W = srk ? T(0.0) : B;
-    return zroot(-(1.0 + B - U), A - B * U - U - W * W, -(A * B - B * W * W - W * W), gas);
+    return zroot_k(-(1.0 + B - U), A - B * U - U - W * W, -(A * B - B * W * W - W * W), gas, zk);
 }
 
 // ------------------------------------------------------- cell properties
@@ -201,8 +216,9 @@ __device__ __forceinline__ void table(const double* tb, int nr, const T& s, T& k
 }
 
 template <int NC, class T>
-__device__ __forceinline__ void viscoT(const AsmConsts& K, const T* x, const T& P, bool gas, T& ro, T& mu) {
-    const T Z = zphase<NC, T>(K, x, P, gas);
+__device__ __forceinline__ void viscoT(const AsmConsts& K, const T* x, const T& P, bool gas, T& ro, T& mu,
+                                       double zk = 0.0) {
+    const T Z = zphase<NC, T>(K, x, P, gas, zk);
     ro = P / (Z * (RGAS * K.T));
     T s1 = 0.0, s2 = 0.0, rv = 0.0, tcm = 0.0, pcm = 0.0, mwm = 0.0;
 #pragma unroll
@@ -251,9 +267,11 @@ __device__ __forceinline__ void diffT(const AsmConsts& K, const T* x, const T& r
     }
 }
 
+// zl, zg: the phase roots when already known (forward mode only; 0 = solve)
 template <int NC, class T>
 __device__ __forceinline__ void propsT(const AsmArgs& A, int k, int c, int lab, const T& Po, const T* x,
-                                       const T* y, const T* S, double rpo, double rpg, PropsT<NC, T>& p) {
+                                       const T* y, const T* S, double rpo, double rpg, PropsT<NC, T>& p,
+                                       double zl = 0.0, double zg = 0.0) {
     const AsmConsts& K = *A.K;
     const double phi = A.phi[k * A.n + c];
     const T dP = Po - K.refP;
@@ -300,8 +318,8 @@ __device__ __forceinline__ void propsT(const AsmArgs& A, int k, int c, int lab, 
         p.Pw = Po;
         p.Pg = Po;
     }
-    viscoT<NC, T>(K, x, p.Po, false, p.ro[0], p.mu[0]);
-    viscoT<NC, T>(K, y, p.Pg, true, p.ro[1], p.mu[1]);
+    viscoT<NC, T>(K, x, p.Po, false, p.ro[0], p.mu[0], zl);
+    viscoT<NC, T>(K, y, p.Pg, true, p.ro[1], p.mu[1], zg);
     diffT<NC, T>(K, x, p.ro[0], p.D[0]);
     diffT<NC, T>(K, y, p.ro[1], p.D[1]);
     T ml = 0.0, mg = 0.0;
@@ -319,7 +337,8 @@ __device__ __forceinline__ void propsT(const AsmArgs& A, int k, int c, int lab, 
 // as propsT.
 template <int NC, class T>
 __device__ __forceinline__ void propsT_loc(const AsmArgs& A, int k, int c, int lab, const T& Po, const T* x,
-                                           const T* y, const T* S, double rpo, double rpg, PropsT<NC, T>& p) {
+                                           const T* y, const T* S, double rpo, double rpg, PropsT<NC, T>& p,
+                                           double zl = 0.0, double zg = 0.0) {
     const AsmConsts& K = *A.K;
     const double phi = A.phi[k * A.n + c];
     const T dP = Po - K.refP;
@@ -343,8 +362,17 @@ __device__ __forceinline__ void propsT_loc(const AsmArgs& A, int k, int c, int l
         p.Pg = Po;
     }
     const double rt = RGAS * K.T;
-    p.ro[0] = p.Po / (zphase<NC, T>(K, x, p.Po, false) * rt);
-    p.ro[1] = p.Pg / (zphase<NC, T>(K, y, p.Pg, true) * rt);
+    p.ro[0] = p.Po / (zphase<NC, T>(K, x, p.Po, false, zl) * rt);
+    p.ro[1] = p.Pg / (zphase<NC, T>(K, y, p.Pg, true, zg) * rt);
+}
+
+// the two phase roots of block blk from the property table (Z = P / (rho R T))
+__device__ __forceinline__ void table_roots(const AsmArgs& A, int blk, double& zl, double& zg) {
+    const int64_t s2 = 2 * (int64_t)A.n;
+    const double* q = A.props + blk;
+    const double rt = RGAS * A.K->T;
+    zl = q[2 * s2] / (q[5 * s2] * rt);
+    zg = q[3 * s2] / (q[6 * s2] * rt);
 }
 
 // state of one block as T
@@ -1560,7 +1588,7 @@ __device__ __forceinline__ void seed_tab(const VarsT<NC, double>& v, const doubl
 // (fugacity and constraint rows) by forward mode with unknown j seeded --
 // they read only Po, Pg and the phase densities, so only that part of the
 // property sweep runs (propsT_loc), with the values of the property table
-// -- into Wl laid out [block][column][local row]
+// and its phase roots -- into Wl laid out [block][local row][column]
 template <int NC>
 __global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, double* Wl) {
     using D1 = DualV<1>;
@@ -1582,8 +1610,10 @@ __global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, double* Wl) {
         rpo = A.rho_prev[c];
         rpg = A.rho_prev[n + c];
     }
+    double zl, zg;
+    table_roots(A, blk, zl, zg);
     PropsT<NC, D1> sp;
-    propsT_loc<NC, D1>(A, k, c, lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp);
+    propsT_loc<NC, D1>(A, k, c, lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp, zl, zg);
     const int64_t s2 = 2 * (int64_t)n;
     const double* tab = A.props + blk;
     sp.Po.v = tab[2 * s2];
@@ -1593,10 +1623,10 @@ __global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, double* Wl) {
     RowAcc<NC, D1> R;
     loc_piece<NC, D1>(A, sv, sp, R);
     const int nl = nloc(lab, NC);
-    double* w = Wl + ((int64_t)blk * MAXC + j) * (NC + 3);
+    double* w = Wl + (int64_t)blk * MAXC * (NC + 3) + j;
 #pragma unroll
     for (int r = 0; r < NC + 3; r++)
-        if (r < nl) w[r] = R.loc[r].d[0];
+        if (r < nl) w[r * MAXC] = R.loc[r].d[0];
 }
 
 // One group of RED_GROUP lanes per block: each lane loads its column of the
@@ -1639,10 +1669,10 @@ __global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, 
             for (int r = 0; r < NLM; r++)
                 if (r < nl) col[r] = vals[rowptr[rb + loc_row(lab, NC, r)] + j];
         } else {
-            const double* w = Wl + ((int64_t)blk * JacShape<NC>::MAXC + j) * NLM;
+            const double* w = Wl + (int64_t)blk * JacShape<NC>::MAXC * NLM + j;
 #pragma unroll
             for (int r = 0; r < NLM; r++)
-                if (r < nl) col[r] = w[r];
+                if (r < nl) col[r] = w[r * JacShape<NC>::MAXC];
         }
     }
     group_eliminate<NC>(col, lab, nl, gbase);
@@ -1650,8 +1680,9 @@ __global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, 
 }
 
 // one lane per (block, direction q <= NP): the property sweep in forward
-// mode along the reduced direction q of the block (v_q, or w for q = NP),
-// written with the direction's seeds of the block's variables
+// mode along the reduced direction q of the block (v_q, or w for q = NP;
+// the phase roots from the property table), written with the direction's
+// seeds of the block's variables
 template <int NC>
 __global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* S, const double* t, double* dred) {
     using D1 = DualV<1>;
@@ -1701,8 +1732,10 @@ __global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* S, 
         rpo = A.rho_prev[c];
         rpg = A.rho_prev[n + c];
     }
+    double zl, zg;
+    table_roots(A, blk, zl, zg);
     PropsT<NC, D1> sp;
-    propsT<NC, D1>(A, k, c, sv.lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp);
+    propsT<NC, D1>(A, k, c, sv.lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp, zl, zg);
 #pragma unroll
     for (int f = 0; f < NF; f++) o[f * (NP + 1)] = prop_field<NC, D1>(sp, f).d[0];
 #pragma unroll
