@@ -337,6 +337,48 @@ int osr_flash_adjoint_device(osr_ctx *ctx, int64_t n, const double *T, double T0
  * the stream. */
 int osr_flash_stats(osr_ctx *ctx, int64_t *out, int32_t nout, void *stream);
 
+/* ====================================================== linear solve
+ * BiCGStab with block-Jacobi preconditioning on a block-CSR system with
+ * square blocks of size bs (the reduced (nc+1)^2 system: bs = nc + 1 in
+ * {3, 4, 5, 6, 7, 9, 11, 13}).  Replaces the reference's linear solve inside
+ * SNESSolve -- KSPSetType(SolverType = "bcgs"), PCSetType(PCVPBJACOBI),
+ * KSPSetTolerances(KSPRelTol, KSPAbsTol, KSPDivTol, MaxIters)
+ * (PetscAUX.cpp:628-646, 669; defaults 1e-18, 1e-50, 1e3, 5000 from
+ * Data_Input.cpp:232-266) -- moved onto the reduced system: the
+ * preconditioner inverts the (nc+1)^2 diagonal blocks of the reduced matrix
+ * rather than the variable-size diagonal blocks of the full AIJ matrix.
+ * Convergence is tested on the unpreconditioned residual norm:
+ * |r| <= max(rtol |b|, atol) converged, |r| >= dtol |r0| diverged.
+ *
+ * osr_ksp_setup_device keeps the (device) matrix pointers and inverts the
+ * diagonal blocks; it returns the number of singular diagonal blocks (>= 0,
+ * the solve then yields non-finite values) or < 0.  osr_ksp_solve_device
+ * solves A x = b (x: initial guess if x0_nonzero, else zeroed) and
+ * synchronises the stream once per iteration to test convergence. */
+typedef struct osr_ksp osr_ksp;
+typedef struct osr_ksp_opts {
+    double rtol, atol, dtol;
+    int32_t maxit;
+    int32_t x0_nonzero;
+} osr_ksp_opts;
+#define OSR_KSP_CONVERGED_RTOL 2
+#define OSR_KSP_CONVERGED_ATOL 3
+#define OSR_KSP_DIVERGED_ITS (-3)
+#define OSR_KSP_DIVERGED_DTOL (-4)
+#define OSR_KSP_DIVERGED_BREAKDOWN (-5)
+#define OSR_KSP_DIVERGED_NANORINF (-9)
+typedef struct osr_ksp_result {
+    int32_t iterations;
+    int32_t reason;       /* OSR_KSP_* (the KSPConvergedReason codes) */
+    double rnorm, bnorm, r0norm;
+} osr_ksp_result;
+int osr_ksp_create(int32_t bs, int64_t nbrows, osr_ksp **out);
+int osr_ksp_destroy(osr_ksp *k);
+int osr_ksp_setup_device(osr_ksp *k, const int64_t *browptr, const int32_t *bcols, const double *bvals,
+                         void *stream);
+int osr_ksp_solve_device(osr_ksp *k, const double *b, double *x, const osr_ksp_opts *opts,
+                         osr_ksp_result *res, void *stream);
+
 /* Library version string and the component counts compiled in. */
 const char *osr_version(void);
 int osr_supported_nc(int32_t nc);

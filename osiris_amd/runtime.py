@@ -547,3 +547,76 @@ def phase_ctrl(ctx, T, label, Po, Pg, x, y, S, stream=None):
                                    _ptr(x), _ptr(y), _ptr(S), _ptr(ps), _stream_ptr(stream)),
            "osr_phase_ctrl_device")
     return ps
+
+
+# ------------------------------------------------------------ linear solve
+class _KspOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
+                ("maxit", C.c_int32), ("x0_nonzero", C.c_int32)]
+
+
+class _KspResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("reason", C.c_int32), ("rnorm", C.c_double),
+                ("bnorm", C.c_double), ("r0norm", C.c_double)]
+
+
+# KSPConvergedReason codes returned in the result
+KSP_REASONS = {2: "converged_rtol", 3: "converged_atol", -3: "diverged_its", -4: "diverged_dtol",
+               -5: "diverged_breakdown", -9: "diverged_nanorinf"}
+# the reference's KSP tolerances when the deck sets none (Data_Input.cpp:232-266)
+KSP_REFERENCE_DEFAULTS = dict(rtol=1e-18, atol=1e-50, dtol=1e3, maxit=5000)
+
+
+class Ksp:
+    """BiCGStab + block-Jacobi on a block-CSR system (osr_ksp_*): the linear
+    solve of the reference's SNES (KSPBCGS + PCVPBJACOBI, PetscAUX.cpp:628-646)
+    on the reduced (nc+1)^2 system."""
+
+    def __init__(self, bs, nbrows):
+        L = lib()
+        if not getattr(L, "_ksp_bound", False):
+            L.osr_ksp_create.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]
+            L.osr_ksp_destroy.argtypes = [C.c_void_p]
+            L.osr_ksp_setup_device.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+            L.osr_ksp_solve_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(_KspOpts),
+                                               C.POINTER(_KspResult), C.c_void_p]
+            L._ksp_bound = True
+        h = C.c_void_p()
+        _check(L.osr_ksp_create(int(bs), int(nbrows), C.byref(h)), "osr_ksp_create")
+        self.h, self.bs, self.nbrows = h, int(bs), int(nbrows)
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().osr_ksp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setup(self, browptr, bcols, bvals, stream=None):
+        """Keep the matrix (device tensors) and invert its diagonal blocks;
+        returns the number of singular diagonal blocks."""
+        self._keep = (browptr, bcols, bvals)
+        return _check(lib().osr_ksp_setup_device(self.h, _ptr(browptr), _ptr(bcols), _ptr(bvals),
+                                                 _stream_ptr(stream)), "osr_ksp_setup_device")
+
+    def solve(self, b, x=None, rtol=1e-10, atol=1e-50, dtol=1e3, maxit=1000, stream=None):
+        """Solve A x = b; x (device, optional) is the initial guess.  Returns
+        (x, {iterations, reason, rnorm, bnorm, r0norm})."""
+        import torch
+        n = self.bs * self.nbrows
+        if b.numel() != n:
+            raise ValueError(f"b has {b.numel()} entries, the system has {n}")
+        x0 = x is not None
+        if x is None:
+            x = torch.empty(n, dtype=torch.float64, device="cuda")
+        o = _KspOpts(float(rtol), float(atol), float(dtol), int(maxit), int(x0))
+        r = _KspResult()
+        _check(lib().osr_ksp_solve_device(self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(r),
+                                          _stream_ptr(stream)), "osr_ksp_solve_device")
+        return x, {"iterations": r.iterations, "reason": KSP_REASONS.get(r.reason, r.reason),
+                   "rnorm": r.rnorm, "bnorm": r.bnorm, "r0norm": r.r0norm}

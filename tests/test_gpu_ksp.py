@@ -1,0 +1,196 @@
+"""BiCGStab + block-Jacobi on the reduced (nc+1)^2 system (osr_ksp_*).
+
+The preconditioner is the reference's class (point-block Jacobi, PCVPBJACOBI
+on the full system there, on the reduced system here), and so is its reach:
+on the 4x3x2 three-component grid BiCGStab converges (98 iterations here; the
+full system with variable point-block Jacobi takes 108 in scipy), while on
+the 6x5x3 six-component grid both arrangements stall near 1e-7 of |b| in
+scipy as on the GPU -- block Jacobi does not resolve the elliptic pressure
+coupling (a CPR/AMG-class preconditioner would).
+
+Bars: where it converges (4x3x2), |R dp - b| <= 1e-10 |b|, agreement with a
+sparse direct solve to 1e-5 of max |dp| and of the expanded update with the
+full system's direct solve; on the 6x5x3 and 5x4x3 (nc = 10) systems, the
+GPU iterates equal scipy's BiCGStab with the same right block-Jacobi
+preconditioner after 1, 2 and 3 iterations (1e-8 of max |x|; later on these
+stalling, ill-conditioned systems the two trajectories separate by rounding
+amplification -- 1.6e-8 at 5 iterations on the nc = 10 system, 0.28 at 20
+on the 6x5x3 one), and on a well-conditioned block-tridiagonal system after
+10 and 20 (1e-9); on a block-diagonal system block Jacobi is exact, so the
+solve stops after one
+iteration at the blockwise numpy solution (1e-12); two solves are bitwise
+identical; the iteration cap and a singular diagonal block report their
+reasons."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import torch
+
+from osiris_amd import synth
+from tests.asm_cases import c0_case, frozen, oracle_flash_fn
+
+pytestmark = pytest.mark.gpu
+
+
+def _reduced(gpu, oracle, fl, grid, st):
+    s = frozen(oracle, fl, grid, st)
+    ctx = gpu.Context(fl)
+    asm = gpu.Assembler(ctx, fl, grid)
+    d = asm.device_state(s)
+    _, nrows = asm.layout(d)
+    F = asm.residual(d)
+    red = asm.reduced_direct(d, F)
+    return ctx, asm, d, nrows, F, red
+
+
+def _csr(browptr, bcols, bvals, n):
+    return sp.bsr_matrix((bvals.cpu().numpy(), bcols.cpu().numpy().astype(np.int64), browptr.cpu().numpy()),
+                         shape=(n, n)).tocsr()
+
+
+def test_reduced_solve_converges_where_block_jacobi_does(gpu, oracle):
+    fl, grid, st = c0_case(oracle, nx=4, ny=3, nz=2, seed=1, sw_m=0.25, sw_f=0.05)
+    ctx, asm, d, nrows, F, (browptr, bcols, bvals, brhs, S, t) = _reduced(gpu, oracle, fl, grid, st)
+    np1, nb = fl.nc + 1, 2 * grid.n
+    ksp = gpu.Ksp(np1, nb)
+    assert ksp.setup(browptr, bcols, bvals) == 0
+    b = brhs.reshape(-1).contiguous()
+    dp, info = ksp.solve(b, rtol=1e-12, maxit=2000)
+    torch.cuda.synchronize()
+    print(info, flush=True)
+    assert info["reason"] in ("converged_rtol", "converged_atol"), info
+    R = _csr(browptr, bcols, bvals, nb * np1)
+    bh, x = b.cpu().numpy(), dp.cpu().numpy()
+    assert np.linalg.norm(R @ x - bh) <= 1e-10 * np.linalg.norm(bh)
+    xd = spla.spsolve(R.tocsc(), bh)
+    assert np.abs(x - xd).max() <= 1e-5 * np.abs(xd).max(), np.abs(x - xd).max() / np.abs(xd).max()
+    du = asm.reduced_expand(d, S, t, dp.reshape(nb, np1)).cpu().numpy()
+    rp, cols, vals = asm.jacobian(d)
+    J = sp.csr_matrix((vals.cpu().numpy(), cols.cpu().numpy(), rp.cpu().numpy()), shape=(nrows, nrows))
+    du_full = spla.spsolve(J.tocsc(), -F.cpu().numpy())
+    assert np.abs(du - du_full).max() <= 1e-5 * np.abs(du_full).max()
+    dp2, info2 = ksp.solve(b, rtol=1e-12, maxit=2000)
+    assert torch.equal(dp, dp2) and info2["iterations"] == info["iterations"]
+    ksp.close()
+
+
+def _scipy_bicgstab_iterates(R, b, Minv_blocks, stops):
+    """scipy's BiCGStab (right preconditioner M, true residual) from x0 = 0,
+    the iterate after each count in stops."""
+    Mi = sp.block_diag(list(Minv_blocks), format="csr")
+    M = spla.LinearOperator(R.shape, matvec=lambda v: Mi @ v)
+    out = {}
+    for k in stops:
+        try:
+            x, _ = spla.bicgstab(R, b, rtol=1e-300, atol=0.0, maxiter=k, M=M)
+        except TypeError:
+            x, _ = spla.bicgstab(R, b, tol=1e-300, atol=0.0, maxiter=k, M=M)
+        out[k] = x
+    return out
+
+
+@pytest.mark.parametrize("which", ["pr6_6x5x3", "pr10_5x4x3"])
+def test_iterates_follow_scipy_bicgstab(gpu, oracle, which):
+    nc = 6 if which == "pr6_6x5x3" else 10
+    fl = synth.fluid_pseudo(nc, synth.PR, 3 if nc == 6 else 5)
+    shape, seed = ((6, 5, 3), 4) if nc == 6 else ((5, 4, 3), 6)
+    grid, st = synth.dual_porosity_case(fl, *shape, seed, oracle_flash_fn(oracle, fl), p_range=(8e6, 20e6))
+    ctx, asm, d, nrows, F, (browptr, bcols, bvals, brhs, S, t) = _reduced(gpu, oracle, fl, grid, st)
+    np1, nb = fl.nc + 1, 2 * grid.n
+    R = _csr(browptr, bcols, bvals, nb * np1)
+    br, bc, bv = browptr.cpu().numpy(), bcols.cpu().numpy(), bvals.cpu().numpy()
+    diag = np.stack([bv[k] for i in range(nb) for k in range(br[i], br[i + 1]) if bc[k] == i])
+    b = brhs.reshape(-1).contiguous()
+    ref = _scipy_bicgstab_iterates(R, b.cpu().numpy(), np.linalg.inv(diag), (1, 2, 3))
+    ksp = gpu.Ksp(np1, nb)
+    assert ksp.setup(browptr, bcols, bvals) == 0
+    for k, xr in ref.items():
+        x, info = ksp.solve(b, rtol=1e-300, atol=0.0, maxit=k)
+        assert info["iterations"] == k, info
+        err = np.abs(x.cpu().numpy() - xr).max() / np.abs(xr).max()
+        print(which, k, info["rnorm"] / info["bnorm"], err, flush=True)
+        assert err <= 1e-8, (which, k, err)
+    ksp.close()
+
+
+def _block_tridiag(nbr, bs, seed, diag_weight):
+    g = np.random.Generator(np.random.PCG64(seed))
+    rows, cols, vals = [], [], []
+    for i in range(nbr):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < nbr:
+                rows.append(i)
+                cols.append(j)
+                vals.append(g.normal(size=(bs, bs)) + (diag_weight * bs * np.eye(bs) if i == j else 0))
+    browptr = np.searchsorted(np.array(rows), np.arange(nbr + 1)).astype(np.int64)
+    return browptr, np.array(cols, np.int32), np.array(vals)
+
+
+def test_iterates_follow_scipy_on_a_well_conditioned_system(gpu):
+    nbr, bs = 800, 7
+    browptr, bcols, bvals = _block_tridiag(nbr, bs, 11, 1.5)
+    n = nbr * bs
+    R = sp.bsr_matrix((bvals, bcols.astype(np.int64), browptr), shape=(n, n)).tocsr()
+    g = np.random.Generator(np.random.PCG64(12))
+    b = g.normal(size=n)
+    diag = np.stack([bvals[k] for i in range(nbr) for k in range(browptr[i], browptr[i + 1]) if bcols[k] == i])
+    ref = _scipy_bicgstab_iterates(R, b, np.linalg.inv(diag), (10, 20))
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    ksp = gpu.Ksp(bs, nbr)
+    assert ksp.setup(dev(browptr), dev(bcols), dev(bvals)) == 0
+    for k, xr in ref.items():
+        x, info = ksp.solve(dev(b), rtol=1e-300, atol=0.0, maxit=k)
+        err = np.abs(x.cpu().numpy() - xr).max() / np.abs(xr).max()
+        print("tridiagonal", k, info["rnorm"] / info["bnorm"], err, flush=True)
+        assert info["iterations"] == k and err <= 1e-9, (k, err)
+    ksp.close()
+
+
+def _block_diag(nbr, bs, seed):
+    g = np.random.Generator(np.random.PCG64(seed))
+    blocks = g.normal(size=(nbr, bs, bs)) + bs * np.eye(bs)[None] * g.uniform(0.5, 2.0, size=(nbr, 1, 1))
+    # a permutation so pivoting is exercised
+    for i in range(0, nbr, 3):
+        blocks[i] = blocks[i][::-1].copy()
+    browptr = np.arange(nbr + 1, dtype=np.int64)
+    bcols = np.arange(nbr, dtype=np.int32)
+    return browptr, bcols, blocks
+
+
+@pytest.mark.parametrize("bs", [3, 7, 11, 13])
+def test_block_jacobi_exact_on_block_diagonal(gpu, bs):
+    nbr = 3000
+    browptr, bcols, blocks = _block_diag(nbr, bs, bs)
+    g = np.random.Generator(np.random.PCG64(1))
+    b = g.normal(size=nbr * bs)
+    ksp = gpu.Ksp(bs, nbr)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    assert ksp.setup(dev(browptr), dev(bcols), dev(blocks)) == 0
+    x, info = ksp.solve(dev(b), rtol=1e-12, maxit=50)
+    ref = np.linalg.solve(blocks, b.reshape(nbr, bs)[..., None])[..., 0].reshape(-1)
+    assert info["iterations"] == 1 and info["reason"] == "converged_rtol", info
+    assert np.abs(x.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    ksp.close()
+
+
+def test_reasons(gpu):
+    nbr, bs = 500, 5
+    g = np.random.Generator(np.random.PCG64(3))
+    # block tridiagonal, not solved in two iterations
+    browptr, cols, vals = _block_tridiag(nbr, bs, 3, 4.0)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    ksp = gpu.Ksp(bs, nbr)
+    assert ksp.setup(dev(browptr), dev(cols), dev(vals)) == 0
+    b = dev(g.normal(size=nbr * bs))
+    _, info = ksp.solve(b, rtol=1e-14, maxit=2)
+    assert info["reason"] == "diverged_its" and info["iterations"] == 2, info
+    _, info = ksp.solve(b, rtol=1e-12, maxit=500)
+    assert info["reason"] == "converged_rtol", info
+    ksp.close()
+    # a zero diagonal block is singular
+    blocks = np.array(vals)
+    blocks[browptr[7] + 1] = 0.0            # block (7, 7)
+    ksp = gpu.Ksp(bs, nbr)
+    assert ksp.setup(dev(browptr), dev(cols), dev(blocks)) == 1
+    ksp.close()
