@@ -281,10 +281,13 @@ def newton_leg(rt, synth, torch, args, log):
     nsing = ksp.setup(browptr, bcols, bvals)
     ms_ksp_setup = (time.perf_counter() - t0) * 1e3
     rhs = brhs.reshape(-1).contiguous()
-    ksp.solve(rhs, rtol=1e-300, atol=0.0, maxit=2)
+    # throughput of the Krylov machinery: a fixed iteration count, with the
+    # divergence test off (block Jacobi does not converge on this system;
+    # the reason of a default-tolerance solve is reported beside)
+    _, kdef = ksp.solve(rhs, rtol=1e-6, atol=0.0, maxit=200)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, kinfo = ksp.solve(rhs, rtol=1e-300, atol=0.0, maxit=KSP_ITS)
+    _, kinfo = ksp.solve(rhs, rtol=1e-300, atol=0.0, dtol=1e300, maxit=KSP_ITS)
     torch.cuda.synchronize()
     ms_ksp_it = (time.perf_counter() - t0) * 1e3 / max(1, kinfo["iterations"])
     ksp.close()
@@ -333,6 +336,8 @@ def newton_leg(rt, synth, torch, args, log):
                     "setup_ms": ms_ksp_setup, "singular_diagonal_blocks": nsing,
                     "ms_per_iteration": ms_ksp_it, "iterations_timed": kinfo["iterations"],
                     "relative_residual_after": kinfo["rnorm"] / kinfo["bnorm"],
+                    "solve_rtol_1e-6_maxit_200": {"reason": kdef["reason"], "iterations": kdef["iterations"],
+                                                  "relative_residual": kdef["rnorm"] / kdef["bnorm"]},
                     "bsr_bytes_per_spmv": nnzb_big * (8 * (NEWTON_NC + 1) ** 2 + 4),
                     "spmv_note": "two SpMV and two block-Jacobi applications per iteration",
                 },
