@@ -1542,10 +1542,10 @@ __global__ void k_reduce_expand(AsmArgs A, const double* S, const double* t, con
 //                derivatives along v_q / w, combined from the per-unknown
 //                ones (the property sweep is linear in the seed), and the
 //                direction's seeds of the block's variables;
-//   k_red_own / k_red_nb / k_red_tr  the balance rows along direction q of
-//                the own block / of a face neighbour / of the other
-//                continuum's block, written to the reduced block's column q,
-//                or (q = NP) to a per-slot partial sum;
+//   k_red_own / k_red_nb  the balance rows along direction q of the own
+//                block / of the blocks that depend on it (six face
+//                neighbours, the other continuum's block), written to the
+//                reduced block's column q, or (q = NP) to a per-slot partial;
 //   k_red_rhs    one lane per block: -F_m plus the slot partials.
 
 // unknown column of variable m of a block in variable order (x_0..x_{nc-1},
@@ -1817,56 +1817,14 @@ __global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, 
     for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
 }
 
-// Faces, column-block-major: one lane per (column block d, face f',
-// direction q).  The row block b is d's neighbour across f' and d sits
-// across the opposite face of b; the lanes of one d run together, so d's
-// derivative table is read once while b's values are gathered.
+// Neighbours, column-block-major: one lane per (column block d, direction
+// q).  The lane seeds d along q once -- d's variables and property
+// derivatives in registers -- and walks the row blocks that depend on d:
+// its six face neighbours b (d sits across the opposite face of b) and the
+// other continuum's block of the same cell (the transfer), writing column q
+// of each reduced block (b, d), or (q = NP) the per-slot partial of b.
 template <int NC>
 __global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, const int64_t* browptr,
-                                                int32_t* bcols, double* bvals, double* rpart) {
-    using D1 = DualV<1>;
-    constexpr int NP = NC + 1;
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= (int64_t)2 * A.n * 6 * (NP + 1)) return;
-    const int q = (int)(item % (NP + 1));
-    const int64_t rest = item / (NP + 1);
-    const int fp = (int)(rest % 6);
-    const int d = (int)(rest / 6);
-    const int n = A.n, k = d / n, cd = d % n;
-    const int cb = nb_of(A, geom(A, cd), fp);
-    if (cb < 0 || A.phi[k * n + cb] < 1e-10) return;   // no neighbour / identity row block
-    const int b = k * n + cb;
-    const int f = fp ^ 1;                                // d as seen from b
-    const Geo G = geom(A, cb);
-    VarsT<NC, double> v;
-    load_vars<NC>(A, k, cb, v);
-    PropsT<NC, double> p;
-    load_props<NC>(A, k, cb, p);
-    VarsT<NC, double> vd;
-    load_vars<NC>(A, k, cd, vd);
-    VarsT<NC, D1> sv;
-    seed_tab<NC>(vd, dred + (int64_t)d * JacShape<NC>::RF * (NP + 1) + q, NP + 1, A.pcoeff, sv);
-    PropsT<NC, D1> sp;
-    load_props_dual<NC>(A, dred, d, q, sp, NP + 1, JacShape<NC>::RF);
-    RowAcc<NC, D1> R;
-    const FaceGeo FG = face_geo(A, G, f);
-    face_piece<NC, double, D1>(A, k, G, FG, f, cb, cd, v, p, sv, sp, R);
-    if (q == NP) {
-        const int64_t nb = 2 * (int64_t)n;
-#pragma unroll
-        for (int i = 0; i < NP; i++) rpart[((int64_t)(f + 1) * NP + i) * nb + b] = R.bal[i].d[0];
-        return;
-    }
-    const int64_t bz = browptr[b] + dep_rank(A, k, G, cb, d);
-    if (q == 0) bcols[bz] = d;
-#pragma unroll
-    for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
-}
-
-// Transfer, column-block-major: one lane per (column block d, direction q);
-// the row block is the other continuum's block of the same cell.
-template <int NC>
-__global__ __launch_bounds__(256) void k_red_tr(AsmArgs A, const double* dred, const int64_t* browptr,
                                                 int32_t* bcols, double* bvals, double* rpart) {
     using D1 = DualV<1>;
     constexpr int NP = NC + 1;
@@ -1874,33 +1832,52 @@ __global__ __launch_bounds__(256) void k_red_tr(AsmArgs A, const double* dred, c
     if (item >= (int64_t)2 * A.n * (NP + 1)) return;
     const int q = (int)(item % (NP + 1));
     const int d = (int)(item / (NP + 1));
-    const int n = A.n, kd = d / n, c = d % n, kb = 1 - kd;
-    if (A.phi[kb * n + c] < 1e-10) return;
-    const int b = kb * n + c;
-    const Geo G = geom(A, c);
-    VarsT<NC, double> v;
-    load_vars<NC>(A, kb, c, v);
-    PropsT<NC, double> p;
-    load_props<NC>(A, kb, c, p);
+    const int n = A.n, k = d / n, cd = d % n;
+    const int64_t nb = 2 * (int64_t)n;
     VarsT<NC, double> vd;
-    load_vars<NC>(A, kd, c, vd);
+    load_vars<NC>(A, k, cd, vd);
     VarsT<NC, D1> sv;
     seed_tab<NC>(vd, dred + (int64_t)d * JacShape<NC>::RF * (NP + 1) + q, NP + 1, A.pcoeff, sv);
     PropsT<NC, D1> sp;
     load_props_dual<NC>(A, dred, d, q, sp, NP + 1, JacShape<NC>::RF);
-    RowAcc<NC, D1> R;
-    if (kb == 0) transfer_piece<NC, double, D1>(A, 0, G, c, v, p, sv, sp, R);
-    else transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, v, p, R);
-    if (q == NP) {
-        const int64_t nb = 2 * (int64_t)n;
+    const Geo Gd = geom(A, cd);
+    auto emit = [&](int b, int slot, int kb, const Geo& G, int cb, const RowAcc<NC, D1>& R) {
+        if (q == NP) {
 #pragma unroll
-        for (int i = 0; i < NP; i++) rpart[((int64_t)7 * NP + i) * nb + b] = R.bal[i].d[0];
-        return;
+            for (int i = 0; i < NP; i++) rpart[((int64_t)slot * NP + i) * nb + b] = R.bal[i].d[0];
+            return;
+        }
+        const int64_t bz = browptr[b] + dep_rank(A, kb, G, cb, d);
+        if (q == 0) bcols[bz] = d;
+#pragma unroll
+        for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
+    };
+#pragma unroll 1
+    for (int fp = 0; fp < 6; fp++) {
+        const int cb = nb_of(A, Gd, fp);
+        if (cb < 0 || A.phi[k * n + cb] < 1e-10) continue;   // no neighbour / identity row block
+        const int f = fp ^ 1;                                // d as seen from b
+        const Geo G = geom(A, cb);
+        VarsT<NC, double> v;
+        load_vars<NC>(A, k, cb, v);
+        PropsT<NC, double> p;
+        load_props<NC>(A, k, cb, p);
+        RowAcc<NC, D1> R;
+        const FaceGeo FG = face_geo(A, G, f);
+        face_piece<NC, double, D1>(A, k, G, FG, f, cb, cd, v, p, sv, sp, R);
+        emit(k * n + cb, f + 1, k, G, cb, R);
     }
-    const int64_t bz = browptr[b] + dep_rank(A, kb, G, c, d);
-    if (q == 0) bcols[bz] = d;
-#pragma unroll
-    for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
+    const int kb = 1 - k;
+    if (A.phi[kb * n + cd] >= 1e-10) {
+        VarsT<NC, double> v;
+        load_vars<NC>(A, kb, cd, v);
+        PropsT<NC, double> p;
+        load_props<NC>(A, kb, cd, p);
+        RowAcc<NC, D1> R;
+        if (kb == 0) transfer_piece<NC, double, D1>(A, 0, Gd, cd, v, p, sv, sp, R);
+        else transfer_piece<NC, D1, double>(A, 1, Gd, cd, sv, sp, v, p, R);
+        emit(kb * n + cd, 7, kb, Gd, cd, R);
+    }
 }
 
 template <int NC>
@@ -2119,7 +2096,6 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     const unsigned g1 = (unsigned)((nb + 255) / 256);
     const unsigned gc = (unsigned)((nb * MAXC + 255) / 256);
     const unsigned gq = (unsigned)((nb * (NP + 1) + 255) / 256);
-    const unsigned gn = (unsigned)((nb * 6 * (NP + 1) + 255) / 256);
     hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3(gc), dim3(256), 0, st, A, a->wloc);
     hipLaunchKernelGGL((osr::k_red_local<NC, false>), dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256),
                        0, st, A, (const double*)a->wloc, F, (const int64_t*)nullptr, (const double*)nullptr, S, t);
@@ -2127,9 +2103,7 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
                        a->dred);
     hipLaunchKernelGGL(osr::k_red_own<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, (const double*)S,
                        (const double*)t, browptr, bcols, bvals, a->rpart);
-    hipLaunchKernelGGL(osr::k_red_nb<NC>, dim3(gn), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
-                       bvals, a->rpart);
-    hipLaunchKernelGGL(osr::k_red_tr<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
+    hipLaunchKernelGGL(osr::k_red_nb<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
                        bvals, a->rpart);
     hipLaunchKernelGGL(osr::k_red_rhs<NC>, dim3(g1), dim3(256), 0, st, A, F, (const double*)a->rpart, brhs);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
