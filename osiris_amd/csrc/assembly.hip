@@ -53,6 +53,7 @@ struct AsmConsts {
     double mw[NCMAX], sqmw[NCMAX], vcm3[NCMAX], mu0[NCMAX], tc[NCMAX], pc[NCMAX], par[NCMAX];
     double v53[NCMAX], v23[NCMAX];
     double pdij[NCMAX * NCMAX];   // pDij (Mass_Tran.cpp:41-45), 0 on the diagonal
+    double ipdij[NCMAX * NCMAX];  // 1 / pDij, 0 on the diagonal
     double pdlast[NCMAX];         // pDij of the last k != j (used when x_j == 1)
     double swt[4 * MAXTAB], sgt[4 * MAXTAB];
     int nswt, nsgt;
@@ -249,15 +250,17 @@ __device__ __forceinline__ void diffT(const AsmConsts& K, const T* x, const T& r
     }
     const T rr = ro * s1 / s2;
     const T roRD = (0.99589 + 0.096016 * rr - 0.22035 * rr * rr + 0.032874 * rr * rr * rr) / ro;
+    // sum_k x_k / (pDij roRD) (Mass_Tran.cpp:40-49) as (sum_k x_k / pDij) / roRD,
+    // with 1 / pDij tabulated: one division per component instead of nc
 #pragma unroll
     for (int j = 0; j < NC; j++) {
         T s = 0.0;
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             if (k == j) continue;
-            s += x[k] / (K.pdij[j * NC + k] * roRD);
+            s += x[k] * K.ipdij[j * NC + k];
         }
-        const T dt = (1.0 - x[j]) / (s * 10000.0);
+        const T dt = (1.0 - x[j]) * roRD / (s * 10000.0);
         T dv;
         if (x[j] == 1.0) dv = roRD * K.pdlast[j];
         else if (dt < 0 || s == 0.0) dv = T(0.0);
@@ -2185,6 +2188,7 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
     }
     for (int j = 0; j < nc; j++) {
         K.pdlast[j] = 0;
+        K.ipdij[j * nc + j] = 0;
         for (int k = 0; k < nc; k++) {
             if (j == k) { K.pdij[j * nc + k] = 0; continue; }
             const double Sij = (coldia[j] + coldia[k]) / 2;
@@ -2194,6 +2198,7 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
             K.pdij[j * nc + k] = 188.2922475 * std::sqrt(g->T * (1.0 / fe->MW[j] + 1.0 / fe->MW[k])) /
                                  (osr::RGAS * Oij * Sij * Sij);
             K.pdlast[j] = K.pdij[j * nc + k];
+            K.ipdij[j * nc + k] = 1.0 / K.pdij[j * nc + k];
         }
     }
     for (int r = 0; r < 4 * g->nswt; r++) K.swt[r] = g->swt[r];
