@@ -773,10 +773,12 @@ __device__ __forceinline__ void loc_piece(const AsmArgs& A, const VarsT<NC, T>& 
         const T FFl = flog((Zl + K.d2 * Bl) / (Zl + K.d1 * Bl));
         const T FFg = flog((Zg + K.d2 * Bg) / (Zg + K.d1 * Bg));
         const T lzl = flog(Zl - Bl), lzg = flog(Zg - Bg);
+        // b_i / Cb and 2 t_i / Ca through the reciprocals (one division per phase)
+        const T iCal = 1.0 / Cal, iCbl = 1.0 / Cbl, iCag = 1.0 / Cag, iCbg = 1.0 / Cbg;
 #pragma unroll
         for (int i = 0; i < NC; i++) {
-            const T t1l = K.b[i] / Cbl, t1g = K.b[i] / Cbg;
-            const T EEl = 2.0 * tl[i] / Cal - t1l, EEg = 2.0 * tg[i] / Cag - t1g;
+            const T t1l = K.b[i] * iCbl, t1g = K.b[i] * iCbg;
+            const T EEl = 2.0 * tl[i] * iCal - t1l, EEg = 2.0 * tg[i] * iCag - t1g;
             const T phiL = fexp(t1l * (Zl - 1.0) - lzl + DDl * EEl * FFl);
             const T phiG = fexp(t1g * (Zg - 1.0) - lzg + DDg * EEg * FFg);
             R.loc[i] += v.x[i] * p.Po * phiL - v.y[i] * p.Pg * phiG;
