@@ -303,6 +303,35 @@ def newton_leg(rt, synth, torch, args, log):
     flash_stats = rt.flash_stats(ctx)
     labels = b.fout["label"].cpu().numpy()
     nnzb_all = sum(int(it["red"][1].shape[0]) for it in b.items)
+    # full-size property: a replay equals an eager evaluation bit for bit
+    # (deterministic, atomic-free kernels), every grid and the batched flash.
+    # Checksum on the device over the raw bits (int64 sums wrap mod 2^64,
+    # so the order of summation does not matter): the plain sum and a sum
+    # weighted by an odd multiplier per position, which changes whenever
+    # any single element changes.
+    def bits_checksum(t):
+        flat = t.reshape(-1)
+        if flat.dtype == torch.float64:
+            flat = flat.view(torch.int64)
+        s1 = torch.zeros((), dtype=torch.int64, device=flat.device)
+        s2 = torch.zeros_like(s1)
+        ch = 1 << 27
+        for i in range(0, flat.numel(), ch):
+            c = flat[i:i + ch].to(torch.int64)
+            w = torch.arange(i, i + c.numel(), device=c.device, dtype=torch.int64) * 5308871522 + 1
+            s1 += c.sum()
+            s2 += (c * w).sum()
+        return int(s1.item()), int(s2.item())
+
+    def digest():
+        return [bits_checksum(t) for t in [b.fout[k] for k in ("x", "y", "L", "label")] +
+                [x for it in b.items for x in (it["F"], it["psat"], *it["red"])]]
+    b.replay()
+    torch.cuda.synchronize()
+    d_replay = digest()
+    b.run_eager()
+    torch.cuda.synchronize()
+    replay_equals_eager = digest() == d_replay
     b.close()
     del b
     torch.cuda.empty_cache()
@@ -343,6 +372,7 @@ def newton_leg(rt, synth, torch, args, log):
                 },
             },
             "frac_two_phase_after_flash": float((labels == 0).mean()),
+            "full_size_replay_equals_eager_bitwise": replay_equals_eager,
             "flash_stats": {k: int(v) for k, v in flash_stats.items()},
         },
         "roofline": {
