@@ -379,6 +379,70 @@ int osr_ksp_setup_device(osr_ksp *k, const int64_t *browptr, const int32_t *bcol
 int osr_ksp_solve_device(osr_ksp *k, const double *b, double *x, const osr_ksp_opts *opts,
                          osr_ksp_result *res, void *stream);
 
+/* ====================================================== Newton step
+ * One time step's nonlinear solve as the reference runs it: SolveNonLinear
+ * (PetscAUX.cpp:566-730) with SNESSetType(SNESKSPONLY) -- one Jacobian at X0,
+ * one linear solve J Y = F and the step X1 = X0 - Y -- its acceptance test
+ * (PetscAUX.cpp:703), and for an accepted step UpdateAll(X, true)
+ * (PetscAUX.cpp:176-507): X -> state (osr_unpack_device's clamps and
+ * limiter), PhaseCtrl and fPhaseCtrl with the gas pressures of the last
+ * property sweep (that of X0), CalcPCoeff (Init_Calcs.cpp:1118-1130), PJac
+ * and the property sweep of the new state.
+ *
+ * On the device: InitGuess2 packs X0, the residual and the reduced
+ * (nc+1)^2 system are assembled straight from the state
+ * (osr_jacobian_bsr_reduced_direct_device), BiCGStab with block Jacobi
+ * solves it (osr_ksp_*, opts->ksp as KSPSetTolerances), and the full update
+ * du = -Y is expanded from its solution.  Accepted, as PetscAUX.cpp:703,
+ * when the SNES reason (SNESKSPONLY: SNES_CONVERGED_ITS = 5 when the linear
+ * solve converged, SNES_DIVERGED_LINEAR_SOLVE = -3 otherwise) and the KSP
+ * reason are positive and vnorm = |Y|_1 / MatrixSize is finite and at most
+ * opts->tolmain (TOLMAIN, Functions.h:241, 1e-2).  A rejected step leaves
+ * the state as it was (the caller's time-step control then shortens Dt, as
+ * TimeStepCtrl does after SolveNonLinear returns 0).  An accepted step
+ * updates s in place, s->pcoeff included, and writes to rho_next (if not
+ * NULL) the matrix oil/gas densities [2n] of the new state's sweep: the
+ * rho_prev_m of the next step (CalcIFT reads them); rho_next may alias
+ * s->rho_prev_m.  pre_m/pre_f must be NULL (Pre_Mul recomputes preProp in
+ * every sweep, so the accumulation of X0 is the reference's).
+ *
+ * osr_newton_create allocates the step's vectors for the assembler's grid
+ * (the reduced blocks on the first step).  osr_newton_solution_update
+ * copies the last step's update du [*nrows] into du (device, may be NULL
+ * to query *nrows), asynchronously on the stream: X1 = X0 + du
+ * (the reference's SNESGetSolutionUpdate, PetscAUX.cpp:681, is -du).  A
+ * step synchronises the stream (layout counts, once per BiCGStab
+ * iteration, the acceptance test). */
+typedef struct osr_newton osr_newton;
+typedef struct osr_newton_opts {
+    osr_ksp_opts ksp;        /* KSPRelTol, KSPAbsTol, KSPDivTol, MaxIters */
+    double tolmain;          /* TOLMAIN */
+} osr_newton_opts;
+#define OSR_SNES_CONVERGED_ITS 5
+#define OSR_SNES_DIVERGED_LINEAR_SOLVE (-3)
+typedef struct osr_newton_result {
+    int32_t accepted;        /* SolveNonLinear's return value: 1 state updated, 0 not */
+    int32_t snes_reason;     /* OSR_SNES_* (PETSc's SNESConvergedReason codes) */
+    int32_t singular_blocks; /* singular diagonal blocks of the reduced system */
+    int32_t reserved;
+    osr_ksp_result ksp;
+    double vnorm;            /* |du|_1 / nrows */
+    int64_t nrows;           /* MatrixSize of the step (the old labels) */
+    int64_t nrows_next;      /* MatrixSize of the new state (PJac); nrows if rejected */
+} osr_newton_result;
+/* Cell state updated in place by the step (labels included). */
+typedef struct osr_state_rw_dev {
+    int8_t *label_m, *label_f;
+    double *Po_m, *Po_f, *x_m, *y_m, *x_f, *y_f, *S_m, *S_f;
+    const double *rho_prev_m;
+    double pcoeff;
+} osr_state_rw_dev;
+int osr_newton_create(osr_asm *a, osr_newton **out);
+int osr_newton_destroy(osr_newton *w);
+int osr_newton_step_device(osr_newton *w, osr_state_rw_dev *s, const osr_newton_opts *opts, double *rho_next,
+                           osr_newton_result *res, void *stream);
+int osr_newton_solution_update(osr_newton *w, double *du, int64_t *nrows, void *stream);
+
 /* Library version string and the component counts compiled in. */
 const char *osr_version(void);
 int osr_supported_nc(int32_t nc);

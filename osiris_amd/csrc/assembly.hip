@@ -2625,3 +2625,44 @@ int osr_pack_device(osr_asm* a, const osr_state_dev* s, double* X, void* stream)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ for the Newton step (newton.hip)
+namespace osr {
+
+void asm_info(const osr_asm* a, osr_ctx** ctx, int64_t* n, int* nc, double* T) {
+    *ctx = a->ctx;
+    *n = a->n;
+    *nc = a->nc;
+    *T = a->hK.T;
+}
+
+// The property sweep of a state into the assembler's table (UpdateDependencies'
+// sweep without a residual); rho_out [2n] (may be null) receives the matrix
+// oil/gas densities of it, the densities the next sweep's CalcIFT reads.
+int asm_sweep(osr_asm* a, const osr_state_dev* s, double* rho_out, hipStream_t st) {
+    const int rc0 = check_state(a, s);
+    if (rc0) return rc0;
+    osr::AsmArgs A = make_args(a, s);
+    int rc = OSR_E_NC;
+    switch (a->nc) {
+        case 2: rc = run_props<2>(a, A, st); break;
+        case 3: rc = run_props<3>(a, A, st); break;
+        case 4: rc = run_props<4>(a, A, st); break;
+        case 5: rc = run_props<5>(a, A, st); break;
+        case 6: rc = run_props<6>(a, A, st); break;
+        case 8: rc = run_props<8>(a, A, st); break;
+        case 10: rc = run_props<10>(a, A, st); break;
+        case 12: rc = run_props<12>(a, A, st); break;
+        default: break;
+    }
+    if (rc) return rc;
+    return rho_out ? asm_take_rho(a, rho_out, st) : OSR_OK;
+}
+
+// The matrix oil/gas densities of the table as the last sweep left it.
+int asm_take_rho(osr_asm* a, double* rho_out, hipStream_t st) {
+    hipLaunchKernelGGL(osr::k_take_rho, dim3((a->n + 255) / 256), dim3(256), 0, st, a->props, a->n, rho_out);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+}  // namespace osr

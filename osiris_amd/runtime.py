@@ -620,3 +620,100 @@ class Ksp:
                                           _stream_ptr(stream)), "osr_ksp_solve_device")
         return x, {"iterations": r.iterations, "reason": KSP_REASONS.get(r.reason, r.reason),
                    "rnorm": r.rnorm, "bnorm": r.bnorm, "r0norm": r.r0norm}
+
+
+# ------------------------------------------------------------ Newton step
+class _NewtonOpts(C.Structure):
+    _fields_ = [("ksp", _KspOpts), ("tolmain", C.c_double)]
+
+
+class _NewtonResult(C.Structure):
+    _fields_ = [("accepted", C.c_int32), ("snes_reason", C.c_int32), ("singular_blocks", C.c_int32),
+                ("reserved", C.c_int32), ("ksp", _KspResult), ("vnorm", C.c_double), ("nrows", C.c_int64),
+                ("nrows_next", C.c_int64)]
+
+
+class _StateRw(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("label_m", "label_f", "Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f",
+                                          "S_m", "S_f", "rho_prev_m")] + [("pcoeff", C.c_double)]
+
+
+# SNESConvergedReason codes of a SNESKSPONLY step
+SNES_REASONS = {5: "converged_its", -3: "diverged_linear_solve"}
+# TOLMAIN (Functions.h:241): largest accepted mean |update| per unknown
+TOLMAIN = 1e-2
+
+
+class NewtonStep:
+    """One time step's nonlinear solve of the reference (SolveNonLinear,
+    PetscAUX.cpp:566-730: SNESKSPONLY -- one Jacobian, one BiCGStab solve,
+    X1 = X0 - Y -- the vnorm <= TOLMAIN acceptance, and for an accepted step
+    UpdateAll(X, true) with PhaseCtrl, CalcPCoeff and PJac) on the device
+    (osr_newton_*), on the reduced (nc+1)^2 system."""
+
+    def __init__(self, asm):
+        L = lib()
+        if not getattr(L, "_newton_bound", False):
+            L.osr_newton_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+            L.osr_newton_destroy.argtypes = [C.c_void_p]
+            L.osr_newton_step_device.argtypes = [C.c_void_p, C.POINTER(_StateRw), C.POINTER(_NewtonOpts),
+                                                 C.c_void_p, C.POINTER(_NewtonResult), C.c_void_p]
+            L.osr_newton_solution_update.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]
+            L._newton_bound = True
+        h = C.c_void_p()
+        _check(L.osr_newton_create(asm.h, C.byref(h)), "osr_newton_create")
+        self.h, self.asm = h, asm
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().osr_newton_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, d, tolmain=TOLMAIN, rtol=KSP_REFERENCE_DEFAULTS["rtol"], atol=KSP_REFERENCE_DEFAULTS["atol"],
+             dtol=KSP_REFERENCE_DEFAULTS["dtol"], maxit=KSP_REFERENCE_DEFAULTS["maxit"], x0_nonzero=False,
+             stream=None):
+        """Run one step on the device state d (Assembler.device_state), in
+        place when accepted: arrays, d["pcoeff"] and d["rho_prev_m"] (the new
+        state's matrix densities, for the next step).  d["pre_m"]/["pre_f"]
+        must be None.  Returns the result as a dict."""
+        import torch
+        if d.get("pre_m") is not None or d.get("pre_f") is not None:
+            raise ValueError("the step recomputes preProp per sweep: pre_m/pre_f must be None")
+        s = _StateRw(*[None if d.get(k) is None else d[k].data_ptr()
+                       for k in ("label_m", "label_f", "Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f", "S_m", "S_f",
+                                 "rho_prev_m")], float(d["pcoeff"]))
+        rho_next = d.get("rho_prev_m")
+        if rho_next is None:
+            rho_next = torch.empty(2 * self.asm.n, dtype=torch.float64, device="cuda")
+        o = _NewtonOpts(_KspOpts(float(rtol), float(atol), float(dtol), int(maxit), int(bool(x0_nonzero))),
+                        float(tolmain))
+        r = _NewtonResult()
+        _check(lib().osr_newton_step_device(self.h, C.byref(s), C.byref(o), _ptr(rho_next), C.byref(r),
+                                            _stream_ptr(stream)), "osr_newton_step_device")
+        if r.accepted:
+            d["pcoeff"] = s.pcoeff
+            d["rho_prev_m"] = rho_next
+        self.asm.nrows = int(r.nrows_next)
+        return {"accepted": bool(r.accepted), "snes_reason": SNES_REASONS.get(r.snes_reason, r.snes_reason),
+                "singular_blocks": r.singular_blocks, "vnorm": r.vnorm, "nrows": r.nrows,
+                "nrows_next": r.nrows_next,
+                "ksp": {"iterations": r.ksp.iterations, "reason": KSP_REASONS.get(r.ksp.reason, r.ksp.reason),
+                        "rnorm": r.ksp.rnorm, "bnorm": r.ksp.bnorm, "r0norm": r.ksp.r0norm}}
+
+    def solution_update(self, stream=None):
+        """A copy of the last step's update du (device tensor, X1 = X0 + du)."""
+        import torch
+        n = C.c_int64()
+        L = lib()
+        _check(L.osr_newton_solution_update(self.h, None, C.byref(n), _stream_ptr(stream)),
+               "osr_newton_solution_update")
+        du = torch.empty(int(n.value), dtype=torch.float64, device="cuda")
+        _check(L.osr_newton_solution_update(self.h, _ptr(du), C.byref(n), _stream_ptr(stream)),
+               "osr_newton_solution_update")
+        return du
