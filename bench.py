@@ -171,6 +171,9 @@ NEWTON_BIG = (256, 256, 32)
 # more than the FP64 passes it replaces (both are timed).
 FAST_FLASH = "persistent"
 KSP_ITS = 20   # BiCGStab iterations timed on the reduced system of the 256x256x32 grid
+STEP_DT = 1.0       # time step (s) of the device time-step measurement
+STEP_RTOL = 1e-8    # its BiCGStab tolerance
+STEP_MAXIT = 300
 
 
 def newton_shapes(seed):
@@ -297,6 +300,32 @@ def newton_leg(rt, synth, torch, args, log):
     torch.cuda.empty_cache()
     log(f"newton: analytic {ms_mixed_big:.1f} ms on the big grid, reduced assembly {ms_red:.1f} ms")
 
+    # one whole time step on the device (SolveNonLinear + UpdateAll(X, true)):
+    # assembly, BiCGStab, acceptance, UpdateAll, PhaseCtrl, PCoeff, PJac and
+    # the new sweep, on the 256x256x32 grid with a 1 s step (block Jacobi
+    # converges on the accumulation-dominated system)
+    import copy as _copy
+    grid_s = _copy.copy(big[0])
+    grid_s.dt = STEP_DT
+    asm_s = rt.Assembler(ctx, fl, grid_s)
+    stepper = rt.NewtonStep(asm_s)
+    step_runs = []
+    for _ in range(2):      # the first allocates the reduced blocks
+        d_s = asm_s.device_state(big[1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sres = stepper.step(d_s, rtol=STEP_RTOL, maxit=STEP_MAXIT)
+        torch.cuda.synchronize()
+        step_runs.append(((time.perf_counter() - t0) * 1e3, sres))
+        del d_s
+    ms_step, sres = step_runs[-1]
+    stepper.close()
+    asm_s.close()
+    del stepper, asm_s
+    torch.cuda.empty_cache()
+    log(f"newton: device time step {ms_step:.1f} ms on the big grid ({sres['ksp']['iterations']} BiCGStab "
+        f"iterations, {sres['ksp']['reason']}, accepted {sres['accepted']})")
+
     # all grids in one graph (headline)
     b = NewtonBatch(ctx, fl, cases, form="reduced", flash_mode=FAST_FLASH)
     ms_all, wall_all = time_graph(b, torch, args.steps, args.warmup)
@@ -369,6 +398,17 @@ def newton_leg(rt, synth, torch, args, log):
                                                   "relative_residual": kdef["rnorm"] / kdef["bnorm"]},
                     "bsr_bytes_per_spmv": nnzb_big * (8 * (NEWTON_NC + 1) ** 2 + 4),
                     "spmv_note": "two SpMV and two block-Jacobi applications per iteration",
+                },
+                "device_time_step": {
+                    "what": "SolveNonLinear (SNESKSPONLY) + UpdateAll(X, true) on the device "
+                            "(osr_newton_step_device): pack, residual, reduced system, BiCGStab + block "
+                            "Jacobi, expansion, acceptance (vnorm <= TOLMAIN), UpdateAll, PhaseCtrl, "
+                            "CalcPCoeff, PJac, new sweep; wall time, host syncs included",
+                    "dt_s": STEP_DT, "ksp_rtol": STEP_RTOL, "ksp_maxit": STEP_MAXIT,
+                    "ms": ms_step, "first_call_ms": step_runs[0][0], "cells_per_s": nbig / (ms_step * 1e-3),
+                    "accepted": sres["accepted"], "snes_reason": sres["snes_reason"], "vnorm": sres["vnorm"],
+                    "ksp_iterations": sres["ksp"]["iterations"], "ksp_reason": sres["ksp"]["reason"],
+                    "ksp_relative_residual": sres["ksp"]["rnorm"] / max(sres["ksp"]["bnorm"], 1e-300),
                 },
             },
             "frac_two_phase_after_flash": float((labels == 0).mean()),
