@@ -72,6 +72,35 @@ def read_traffic(path):
     return None
 
 
+# VALU issue ceiling: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 VALU
+# instruction per SIMD every 4 cycles
+PEAK_VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def valu_view(path, ms):
+    """VALU issue rate and FP64 rate of a launch from a committed rocprofv3
+    --pmc summary (wave64 VALU instructions and FP64 flops per launch) and
+    the launch time measured here."""
+    got = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                if "=" in line and line.split("=")[0] in ("valu_wave_instructions_per_launch", "fp64_flop_per_launch"):
+                    k, v = line.strip().split("=")
+                    got[k] = float(v)
+    except OSError:
+        return None
+    if len(got) < 2:
+        return None
+    ins = got["valu_wave_instructions_per_launch"] / (ms * 1e-3)
+    fl = got["fp64_flop_per_launch"] / (ms * 1e-3) / 1e12
+    return {"wave_instructions_per_launch": got["valu_wave_instructions_per_launch"],
+            "issue_rate_per_s": ins, "issue_peak_per_s": PEAK_VALU_WAVE_INSTR_PER_S,
+            "issue_frac": ins / PEAK_VALU_WAVE_INSTR_PER_S,
+            "fp64_flop_per_launch": got["fp64_flop_per_launch"], "fp64_tflops": fl,
+            "fp64_peak_tflops": PEAK_FP64_TFLOPS, "fp64_frac": fl / PEAK_FP64_TFLOPS}
+
+
 def assembly_leg(rt, synth, torch, args):
     """The differentiation workload: 64x64x8 dual-porosity grid with 6
     components (65,536 cells per continuum): residual + reference-layout CSR
@@ -426,6 +455,9 @@ def newton_leg(rt, synth, torch, args, log):
             "algorithmic_bytes_per_launch": red_bytes,
             "avg_launch_ms": ms_red,
             "traffic": read_traffic(os.path.join(ROOT, "profiles", "reduced_assembly_pr10_256x256x32_pmc.txt")),
+            # what actually bounds these kernels: VALU issue (counters of the
+            # same launch, committed profile), against this run's launch time
+            "valu": valu_view(os.path.join(ROOT, "profiles", "reduced_assembly_pr10_256x256x32_valu.txt"), ms_red),
         },
     }
 
