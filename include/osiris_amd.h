@@ -443,6 +443,35 @@ int osr_newton_step_device(osr_newton *w, osr_state_rw_dev *s, const osr_newton_
                            osr_newton_result *res, void *stream);
 int osr_newton_solution_update(osr_newton *w, double *du, int64_t *nrows, void *stream);
 
+/* ====================================================== restart file
+ * The reference's restart file (WriteRST / ReadRST, Do_Time_Step.cpp:179-294,
+ * "test.rst") as a byte image in device memory, byte for byte the file:
+ * P[Nx+2][Ny+2][Nz+2][3] and fP (water, oil, gas pressure; the ghost halo
+ * holds 0), then per matrix cell and per fracture cell, Ix outer and Iz
+ * inner, sat[3], (comp[i][0], comp[i][1]) for i < nc and phaseStat (int8),
+ * then simTime, Dt (double), TSMCtrl (int32), SMethQ (double); little-endian,
+ * unpadded.  osr_rst_bytes gives the image size (= the file size).
+ * osr_rst_write_device takes the state and the property table of its sweep
+ * for the water and gas pressures (osr_asm_props_table after a residual,
+ * Jacobian or accepted step, or the output of osr_props_device).
+ * osr_rst_read_device fills the state (labels, Po, x, y, S) and the scalars
+ * (it synchronises the stream) and, when phase_p is not NULL, the water and
+ * gas pressures [4][n] (Pw_m, Pg_m, Pw_f, Pg_f), which the reference reads
+ * and then recomputes (UpdateDependencies after ReadRST, PrimeSim.cpp:84-85).
+ * img must be 8-byte aligned; moving the image to and from the file is the
+ * caller's (one contiguous transfer). */
+typedef struct osr_rst_scalars {
+    double sim_time, dt;     /* simTime, Dt */
+    int32_t tsm_ctrl;        /* TSMCtrl */
+    int32_t reserved;
+    double smethq;           /* SMethQ */
+} osr_rst_scalars;
+int osr_rst_bytes(int32_t nx, int32_t ny, int32_t nz, int32_t nc, int64_t *bytes);
+int osr_rst_write_device(osr_asm *a, const osr_state_dev *s, const double *props, const osr_rst_scalars *sc,
+                         uint8_t *img, void *stream);
+int osr_rst_read_device(osr_asm *a, const uint8_t *img, osr_state_rw_dev *s, double *phase_p, osr_rst_scalars *sc,
+                        void *stream);
+
 /* Library version string and the component counts compiled in. */
 const char *osr_version(void);
 int osr_supported_nc(int32_t nc);

@@ -305,3 +305,53 @@ def residual_scale(fluid, grid, state):
     F = np.zeros(int(off[-1]))
     lib().or_residual_scale(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state), _p(F))
     return F
+
+
+def rst_bytes(nx, ny, nz, nc):
+    lib().or_rst_bytes.restype = C.c_int64
+    return int(lib().or_rst_bytes(nx, ny, nz, nc))
+
+
+def rst_write(nx, ny, nz, nc, P_m, P_f, state, sim_time, dt, tsm_ctrl, smethq):
+    """WriteRST (Do_Time_Step.cpp:179-235) into bytes.  P_m, P_f: [3, n]
+    water, oil, gas pressures; state: labels, S, x, y of both continua."""
+    L = lib()
+    L.or_rst_write.restype = C.c_int64
+    L.or_rst_write.argtypes = [C.c_int] * 4 + [C.c_void_p] * 10 + [C.c_double, C.c_double, C.c_int32,
+                                                                    C.c_double, C.c_void_p]
+    img = np.zeros(rst_bytes(nx, ny, nz, nc), np.uint8)
+    a = {k: np.ascontiguousarray(getattr(state, k), np.int8 if k.startswith("label") else np.float64)
+         for k in ("S_m", "x_m", "y_m", "label_m", "S_f", "x_f", "y_f", "label_f")}
+    P_m, P_f = _c(P_m), _c(P_f)
+    w = L.or_rst_write(nx, ny, nz, nc, _p(P_m), _p(P_f), _p(a["S_m"]), _p(a["x_m"]), _p(a["y_m"]),
+                       a["label_m"].ctypes.data_as(C.c_void_p), _p(a["S_f"]), _p(a["x_f"]), _p(a["y_f"]),
+                       a["label_f"].ctypes.data_as(C.c_void_p), float(sim_time), float(dt), int(tsm_ctrl),
+                       float(smethq), img.ctypes.data_as(C.c_void_p))
+    assert w == img.size
+    return img.tobytes()
+
+
+def rst_read(nx, ny, nz, nc, data):
+    """ReadRST (Do_Time_Step.cpp:237-294) of bytes: dict of P_m, P_f [3, n],
+    S_*, x_*, y_*, label_* and the four scalars."""
+    L = lib()
+    L.or_rst_read.restype = C.c_int64
+    L.or_rst_read.argtypes = [C.c_int] * 4 + [C.c_void_p] * 11 + [C.c_void_p] * 4
+    n = nx * ny * nz
+    img = np.frombuffer(data, np.uint8)
+    assert img.size == rst_bytes(nx, ny, nz, nc)
+    out = {"P_m": np.zeros((3, n)), "P_f": np.zeros((3, n))}
+    for sfx in ("_m", "_f"):
+        out["S" + sfx] = np.zeros((3, n))
+        out["x" + sfx] = np.zeros((nc, n))
+        out["y" + sfx] = np.zeros((nc, n))
+        out["label" + sfx] = np.zeros(n, np.int8)
+    t, d, q = C.c_double(), C.c_double(), C.c_double()
+    ts = C.c_int32()
+    r = L.or_rst_read(nx, ny, nz, nc, img.ctypes.data_as(C.c_void_p), _p(out["P_m"]), _p(out["P_f"]),
+                      _p(out["S_m"]), _p(out["x_m"]), _p(out["y_m"]), out["label_m"].ctypes.data_as(C.c_void_p),
+                      _p(out["S_f"]), _p(out["x_f"]), _p(out["y_f"]), out["label_f"].ctypes.data_as(C.c_void_p),
+                      C.byref(t), C.byref(d), C.byref(ts), C.byref(q))
+    assert r == img.size
+    out.update(sim_time=t.value, dt=d.value, tsm_ctrl=ts.value, smethq=q.value)
+    return out

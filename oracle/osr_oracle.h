@@ -158,6 +158,20 @@ int64_t or_jacobian(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g,
 int or_unpack(const or_grid *g, or_state_mut *s, int nc, const double *X);
 int or_pack(const or_grid *g, const or_state *s, int nc, double *X);
 
+/* ------------------------------------------------------------ restart file */
+/* WriteRST / ReadRST (Do_Time_Step.cpp:179-294) into / out of a memory image
+ * of or_rst_bytes bytes.  P_m, P_f: [3*n] water, oil, gas pressures of the
+ * interior cells (may be NULL when reading); the other arrays as or_state.
+ * Both return the bytes written / read. */
+int64_t or_rst_bytes(int nx, int ny, int nz, int nc);
+int64_t or_rst_write(int nx, int ny, int nz, int nc, const double *P_m, const double *P_f, const double *S_m,
+                     const double *x_m, const double *y_m, const int8_t *label_m, const double *S_f,
+                     const double *x_f, const double *y_f, const int8_t *label_f, double sim_time, double dt,
+                     int32_t tsm_ctrl, double smethq, uint8_t *img);
+int64_t or_rst_read(int nx, int ny, int nz, int nc, const uint8_t *img, double *P_m, double *P_f, double *S_m,
+                    double *x_m, double *y_m, int8_t *label_m, double *S_f, double *x_f, double *y_f,
+                    int8_t *label_f, double *sim_time, double *dt, int32_t *tsm_ctrl, double *smethq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2636,6 +2636,12 @@ void asm_info(const osr_asm* a, osr_ctx** ctx, int64_t* n, int* nc, double* T) {
     *T = a->hK.T;
 }
 
+void asm_dims(const osr_asm* a, int* nx, int* ny, int* nz) {
+    *nx = a->nx;
+    *ny = a->ny;
+    *nz = a->nz;
+}
+
 // The property sweep of a state into the assembler's table (UpdateDependencies'
 // sweep without a residual); rho_out [2n] (may be null) receives the matrix
 // oil/gas densities of it, the densities the next sweep's CalcIFT reads.

@@ -59,6 +59,7 @@ int phase_ctrl(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* labe
 bool nc_supported(int nc);
 // assembly.hip, for the Newton step
 void asm_info(const osr_asm* a, osr_ctx** ctx, int64_t* n, int* nc, double* T);
+void asm_dims(const osr_asm* a, int* nx, int* ny, int* nz);
 int asm_sweep(osr_asm* a, const osr_state_dev* s, double* rho_out, hipStream_t st);
 int asm_take_rho(osr_asm* a, double* rho_out, hipStream_t st);
 }  // namespace osr

@@ -319,6 +319,7 @@ class Assembler:
         self.ctx = ctx
         self.nc = ctx.nc
         self.n = grid.nx * grid.ny * grid.nz
+        self.grid = grid
         keep = []
 
         def P(a):
@@ -717,3 +718,85 @@ class NewtonStep:
         _check(L.osr_newton_solution_update(self.h, _ptr(du), C.byref(n), _stream_ptr(stream)),
                "osr_newton_solution_update")
         return du
+
+
+# ------------------------------------------------------------ restart file
+class _RstScalars(C.Structure):
+    _fields_ = [("sim_time", C.c_double), ("dt", C.c_double), ("tsm_ctrl", C.c_int32), ("reserved", C.c_int32),
+                ("smethq", C.c_double)]
+
+
+def _bind_rst(L):
+    if getattr(L, "_rst_bound", False):
+        return
+    L.osr_rst_bytes.argtypes = [C.c_int32] * 4 + [C.POINTER(C.c_int64)]
+    L.osr_rst_write_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.POINTER(_RstScalars),
+                                       C.c_void_p, C.c_void_p]
+    L.osr_rst_read_device.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_StateRw), C.c_void_p,
+                                      C.POINTER(_RstScalars), C.c_void_p]
+    L._rst_bound = True
+
+
+def rst_bytes(nx, ny, nz, nc):
+    """Size of the reference's restart file (WriteRST) for this grid."""
+    L = lib()
+    _bind_rst(L)
+    b = C.c_int64()
+    _check(L.osr_rst_bytes(nx, ny, nz, nc, C.byref(b)), "osr_rst_bytes")
+    return int(b.value)
+
+
+def rst_write(asm, d, props=None, sim_time=0.0, dt=0.0, tsm_ctrl=0, smethq=0.0, path=None, stream=None):
+    """WriteRST (compsim/Do_Time_Step.cpp:179-235): the restart image of the
+    device state d as a uint8 device tensor (and, with path, the file).
+    props: the property table of d's sweep (default: a fresh sweep)."""
+    import torch
+    L = lib()
+    _bind_rst(L)
+    g = asm.grid
+    if props is None:
+        if asm.nrows is None:
+            asm.layout(d, stream)
+        props = asm.props(d, stream)
+    nb = rst_bytes(g.nx, g.ny, g.nz, asm.nc)
+    img = torch.empty((nb + 7) // 8 * 8, dtype=torch.uint8, device="cuda")
+    sc = _RstScalars(float(sim_time), float(dt), int(tsm_ctrl), 0, float(smethq))
+    sd = asm._sd(d)
+    _check(L.osr_rst_write_device(asm.h, C.byref(sd), _ptr(props), C.byref(sc), _ptr(img), _stream_ptr(stream)),
+           "osr_rst_write_device")
+    img = img[:nb]
+    if path is not None:
+        (torch.cuda.current_stream() if stream is None else stream).synchronize()
+        img.cpu().numpy().tofile(path)
+    return img
+
+
+def rst_read(asm, d, src, stream=None):
+    """ReadRST (compsim/Do_Time_Step.cpp:237-294): fill the device state d
+    (labels, Po, x, y, S) from a restart image (uint8 device tensor) or file
+    path.  Returns (scalars dict, phase pressures [4, n]: Pw_m, Pg_m, Pw_f,
+    Pg_f)."""
+    import torch
+    L = lib()
+    _bind_rst(L)
+    g = asm.grid
+    nb = rst_bytes(g.nx, g.ny, g.nz, asm.nc)
+    if isinstance(src, (str, os.PathLike)):
+        host = np.fromfile(src, dtype=np.uint8)
+        if host.size != nb:
+            raise ValueError(f"{src}: {host.size} bytes, a restart of this grid has {nb}")
+        img = torch.empty((nb + 7) // 8 * 8, dtype=torch.uint8, device="cuda")
+        img[:nb].copy_(torch.from_numpy(host))
+    else:
+        if src.numel() != nb:
+            raise ValueError(f"image has {src.numel()} bytes, a restart of this grid has {nb}")
+        img = src
+        if img.data_ptr() % 8:
+            img = img.clone()
+    s = _StateRw(*[d[k].data_ptr() for k in ("label_m", "label_f", "Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f",
+                                              "S_m", "S_f")], None, float(d["pcoeff"]))
+    pp = torch.empty(4, asm.n, dtype=torch.float64, device="cuda")
+    sc = _RstScalars()
+    _check(L.osr_rst_read_device(asm.h, _ptr(img), C.byref(s), _ptr(pp), C.byref(sc), _stream_ptr(stream)),
+           "osr_rst_read_device")
+    return {"sim_time": sc.sim_time, "dt": sc.dt, "tsm_ctrl": sc.tsm_ctrl, "smethq": sc.smethq}, pp
