@@ -355,6 +355,32 @@ def newton_leg(rt, synth, torch, args, log):
     log(f"newton: device time step {ms_step:.1f} ms on the big grid ({sres['ksp']['iterations']} BiCGStab "
         f"iterations, {sres['ksp']['reason']}, accepted {sres['accepted']})")
 
+    # the restart file of the 256x256x32 state (WriteRST / ReadRST byte image)
+    asm_r = rt.Assembler(ctx, fl, big[0])
+    d_r = asm_r.device_state(big[1])
+    asm_r.layout(d_r)
+    props_r = asm_r.props(d_r)
+    img = rt.rst_write(asm_r, d_r, props_r)
+    rt.rst_read(asm_r, d_r, img)
+    reps = 5
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        rt.rst_write(asm_r, d_r, props_r)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_rst_w = e0.elapsed_time(e1) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rt.rst_read(asm_r, d_r, img)       # synchronises for the scalars
+    ms_rst_r = (time.perf_counter() - t0) * 1e3 / reps
+    rst_b = int(img.numel())
+    asm_r.close()
+    del asm_r, d_r, props_r, img
+    torch.cuda.empty_cache()
+    log(f"newton: restart image {rst_b / 1e9:.2f} GB written in {ms_rst_w:.2f} ms, read in {ms_rst_r:.2f} ms")
+
     # all grids in one graph (headline)
     b = NewtonBatch(ctx, fl, cases, form="reduced", flash_mode=FAST_FLASH)
     ms_all, wall_all = time_graph(b, torch, args.steps, args.warmup)
@@ -427,6 +453,12 @@ def newton_leg(rt, synth, torch, args, log):
                                                   "relative_residual": kdef["rnorm"] / kdef["bnorm"]},
                     "bsr_bytes_per_spmv": nnzb_big * (8 * (NEWTON_NC + 1) ** 2 + 4),
                     "spmv_note": "two SpMV and two block-Jacobi applications per iteration",
+                },
+                "restart_image": {
+                    "what": "WriteRST / ReadRST byte image of the 256x256x32 state in HBM (osr_rst_write_device, "
+                            "osr_rst_read_device), file transfer excluded",
+                    "bytes": rst_b, "write_ms": ms_rst_w, "read_ms_wall": ms_rst_r,
+                    "write_GBs_image_plus_state": 2 * rst_b / (ms_rst_w * 1e-3) / 1e9,
                 },
                 "device_time_step": {
                     "what": "SolveNonLinear (SNESKSPONLY) + UpdateAll(X, true) on the device "
