@@ -17,8 +17,11 @@ mixed FP32/FP64 flash are timed beside it on the 256x256x32 grid
 
 Secondary legs: the batched-flash configuration (8 pseudo-components, 2M
 independent cells at +/-15 % of Psat; plain, persistent and mixed forms,
-parity against the oracle) and the 64x64x8 differentiation configuration
-(permeability adjoint, flash adjoint).  The CPU baseline is the oracle (FP64
+parity against the oracle), the mixed-precision configuration (12
+pseudo-components, SRK, 4M independent cells; plain, persistent and mixed
+forms, K-values, phase fractions and ln-fugacities per cell against the
+oracle) and the 64x64x8 differentiation configuration (permeability
+adjoint, flash adjoint).  The CPU baseline is the oracle (FP64
 C/C++ restatement) on bounded samples of the same workloads.
 
 Prints ONE JSON line (rank 0).  Only one GPU is used; --gpus > 1 exits.
@@ -566,6 +569,125 @@ def flash_leg(rt, synth, torch, args):
     return res, fl, cells, gpu_res, mixed_res
 
 
+SRK_NC = 12
+SRK_CELLS = 4_000_000
+SRK_FLUID_SEED = 2
+SRK_CELL_SEED = 11
+
+
+def srk_mixed_leg(rt, synth, torch, args):
+    """The mixed-precision configuration: 12 pseudo-components, SRK,
+    4,000,000 independent cells at +/-15 % of Psat; the plain and persistent
+    FP64 forms and the mixed FP32 SS + FP64 root/RR/Newton-polish form, and
+    the mixed result against the FP64 GPU result on every cell."""
+    dev = torch.device("cuda:0")
+    fl = synth.fluid_pseudo(SRK_NC, synth.SRK, SRK_FLUID_SEED)
+    ctx = rt.Context(fl)
+    n = SRK_CELLS
+    cells = synth.flash_cells(fl, n, SRK_CELL_SEED, rt.psat_numpy(ctx))
+    T = torch.tensor(cells.T, device=dev)
+    P = torch.tensor(cells.P, device=dev)
+    z = torch.tensor(cells.z, device=dev)
+    stream = torch.cuda.current_stream()
+    steps = max(3, args.steps // 2)
+
+    def run(mode, out, steps):
+        rt.flash(ctx, T, P, z, mode=mode, out=out)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+        for s_, e in ev:
+            s_.record(stream)
+            rt.flash(ctx, T, P, z, mode=mode, out=out)
+            e.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([s_.elapsed_time(e) for s_, e in ev]))
+
+    keys = ("x", "y", "K", "L", "label", "iters")
+    res = {}
+    ms = {}
+    for mode, k in (("plain", max(3, steps // 4)), ("persistent", steps), ("mixed", steps)):
+        out = rt.alloc_flash_out(SRK_NC, n)
+        ms[mode] = run(mode, out, k)
+        if mode == "mixed":
+            ms["mixed_stats"] = rt.flash_stats(ctx)
+        res[mode] = {key: out[key].cpu().numpy() for key in keys}
+        del out
+    a, b = res["mixed"], res["persistent"]
+    conv = (a["iters"] < 500) & (b["iters"] < 500)
+    two = (b["label"] == 0) & conv
+
+    def rel(u, v):
+        e = np.abs(u - v) / np.maximum(np.abs(v), 1e-12)
+        return np.where(np.isfinite(e), e, 0.0)
+
+    def stats(e):
+        return {"max": float(e.max()) if e.size else 0.0,
+                "p99": float(np.quantile(e, 0.99)) if e.size else 0.0}
+
+    vs_fp64 = {"label_mismatches": int((a["label"] != b["label"]).sum()),
+               "cells_compared": int(conv.sum()),
+               "K_rel_per_cell": stats(rel(a["K"], b["K"]).max(axis=0)[two]),
+               "L_rel_two_phase": stats(rel(a["L"], b["L"])[two])}
+    detail = {
+        "workload": f"mixed-precision flash: {SRK_NC} pseudo-components, SRK, {n} independent cells, "
+                    "p within +/-15% of Psat",
+        "n_cells": n,
+        "plain_fp64": {"cells_per_s": n / (ms["plain"] * 1e-3), "kernel_ms": ms["plain"]},
+        "persistent_fp64": {"cells_per_s": n / (ms["persistent"] * 1e-3), "kernel_ms": ms["persistent"]},
+        "mixed_fp32_fp64": {"cells_per_s": n / (ms["mixed"] * 1e-3), "kernel_ms": ms["mixed"],
+                            "stages": ms["mixed_stats"]},
+        "speedup_mixed_vs_plain": ms["plain"] / ms["mixed"],
+        "speedup_persistent_vs_plain": ms["plain"] / ms["persistent"],
+        "frac_two_phase": float((b["label"] == 0).mean()),
+        "mixed_vs_gpu_fp64_every_cell": vs_fp64,
+    }
+    return detail, fl, cells, res
+
+
+def srk_oracle_parity(O, fl, cells, res, threads):
+    """Every cell of the mixed-precision workload against the FP64 oracle:
+    labels, K-values and phase fractions per cell, and the ln-fugacities
+    ln(x_i phi_i P) of both phases on a sample of two-phase cells."""
+    t0 = time.perf_counter()
+    ref = O.flash_batch(fl, cells.T, cells.P, cells.z, nthreads=threads)
+    cpu_s = time.perf_counter() - t0
+    out = {"oracle_s": cpu_s, "oracle_threads": threads}
+    for mode in ("mixed", "persistent"):
+        r = res[mode]
+        conv = (ref["iters"] < 500) & (r["iters"] < 500)
+        two = conv & (ref["label"] == 0)
+        eK = np.abs(r["K"] - ref["K"]) / np.maximum(np.abs(ref["K"]), 1e-300)
+        eK = np.where(np.isfinite(eK), eK, 0.0).max(axis=0)
+        eL = np.abs(r["L"] - ref["L"]) / np.maximum(np.abs(ref["L"]), 1e-300)
+        rng = np.random.default_rng(0)
+        idx = np.flatnonzero(two)
+        idx = rng.choice(idx, size=min(2000, idx.size), replace=False) if idx.size else idx
+        lnf = []
+        for c in idx:
+            T, P = float(cells.T[c]), float(cells.P[c])
+            worst = 0.0
+            for comp, gas in (("x", 0), ("y", 1)):
+                _, l_mine = O.phase(fl, T, P, r[comp][:, c], gas)
+                _, l_ref = O.phase(fl, T, P, ref[comp][:, c], gas)
+                f_mine = np.log(r[comp][:, c]) + l_mine
+                f_ref = np.log(ref[comp][:, c]) + l_ref
+                worst = max(worst, float(np.max(np.abs(f_mine - f_ref))))
+            lnf.append(worst)
+        lnf = np.array(lnf)
+        out[mode] = {
+            "label_mismatches": int((r["label"] != ref["label"]).sum()),
+            "cells_capped_either_side": int((~conv).sum()),
+            "K_rel_per_cell_two_phase": {"max": float(eK[two].max()), "p99": float(np.quantile(eK[two], 0.99)),
+                                         "frac_le_1e-8": float((eK[two] <= 1e-8).mean())},
+            "L_rel_two_phase": {"max": float(eL[two].max()), "p99": float(np.quantile(eL[two], 0.99)),
+                                "frac_le_1e-8": float((eL[two] <= 1e-8).mean())},
+            "ln_fugacity_abs_sample": {"cells": int(idx.size), "max": float(lnf.max()) if lnf.size else 0.0,
+                                       "p99": float(np.quantile(lnf, 0.99)) if lnf.size else 0.0},
+        }
+    return out
+
+
 def cpu_newton_sample(rt, synth, torch):
     """CPU baseline of the Newton-iteration step: the oracle's flash of every
     cell, PhaseCtrl of both continua, residual and reference-layout Jacobian
@@ -643,6 +765,7 @@ def main():
     ap.add_argument("--no-flash", action="store_true", help="skip the batched-flash leg")
     ap.add_argument("--no-assembly", action="store_true", help="skip the 64x64x8 sensitivity leg")
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton-iteration leg")
+    ap.add_argument("--no-srk", action="store_true", help="skip the 12-component SRK mixed-precision leg")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed legs, copy a tensor of this many MiB once (a "
                          "kernel of known HBM traffic that scales rocprofv3 FETCH_SIZE / "
@@ -687,6 +810,10 @@ def main():
             result["config"] = {"workload": fres["workload"], "n_cells": args.cells, "nc": NC, "eos": "PR",
                                 "parallelism": "single GPU"}
             result["roofline"] = fres["roofline"]
+    if not args.no_srk:
+        sres, sfl, scells, sgpu = srk_mixed_leg(rt, synth, torch, args)
+        result["flash_mixed_srk12"] = sres
+        log("SRK mixed-precision leg done")
     if not args.no_assembly:
         result["sensitivities_64x64x8"] = assembly_leg(rt, synth, torch, args)
         log("sensitivity leg done")
@@ -732,6 +859,10 @@ def main():
                 "single_core_cells_per_s": m / cpu1_s, "single_core_sample": f"first {m} cells, 1 thread"}
             result.setdefault("parity_vs_oracle", {})["flash_persistent_fp64"] = parity(gpu_res)
             result["parity_vs_oracle"]["flash_mixed_fp32_fp64"] = parity(mixed_res)
+        if not args.no_srk:
+            result.setdefault("parity_vs_oracle", {})["flash_mixed_srk12"] = srk_oracle_parity(
+                O, sfl, scells, sgpu, threads)
+            log("SRK oracle parity done")
         if base:
             result["cpu_baseline"] = base
         elif "cpu_baseline" in result.get("flash_batched", {}):
