@@ -559,11 +559,22 @@ __device__ __forceinline__ int nloc(int lab, int nc) { return lab == 0 ? nc + 3 
 // Flux across face q between self (s) and neighbour (o): convective and
 // gravity terms of every component, total and water row, and the Fickian
 // flux of each component (Calc_Transes + IteratorFunction face terms).
-template <int NC, class TS, class TO>
+//
+// MIRROR also accumulates, into mb[0..NC], the same face's terms of the
+// NEIGHBOUR's balance rows.  Every convective, gravity and water term is
+// antisymmetric between the two rows: the potential, the upstream choice,
+// the transmissibility and the mixed gravity are formed in (a, b) order
+// whichever cell is "self", and the pressure and height differences change
+// sign exactly.  The one exception is the Fickian upstream coefficient:
+// each side takes the other's D_i only where the other's composition is
+// strictly larger, so where x_s == x_o each row takes its own D_i; the
+// neighbour's row therefore forms its own coefficient below.
+template <int NC, class TS, class TO, bool MIRROR = false>
 __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G, const FaceGeo& FG, int q, int c,
                                            int o, const VarsT<NC, TS>& vs, const PropsT<NC, TS>& ps,
                                            const VarsT<NC, TO>& vo, const PropsT<NC, TO>& po,
-                                           RowAcc<NC, decltype(TS() + TO())>& R, bool fick = true) {
+                                           RowAcc<NC, decltype(TS() + TO())>& R, bool fick = true,
+                                           decltype(TS() + TO())* mb = nullptr) {
     using T = decltype(TS() + TO());
     const AsmConsts& K = *A.K;
     const int d = q >> 1;
@@ -583,6 +594,7 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
         const T tw = (Kf * (K.wat_ro * WAT_M_RO / K.wat_mu)) * (up_self ? T(ps.kr[0]) : T(po.kr[0]));
         const T fw = tw * (po.Pw - ps.Pw) + G_ACC * K.wat_ro * tw * dh;
         R.bal[NC] -= fw;
+        if constexpr (MIRROR) mb[NC] += fw;
     }
     // oil and gas
 #pragma unroll
@@ -617,11 +629,18 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
         const T base = t * dPp + tg * dh;
         // total hydrocarbon row
         R.bal[NC - 1] -= base;
+        if constexpr (MIRROR) mb[NC - 1] += base;
 #pragma unroll
         for (int i = 0; i < NC - 1; i++) {
             const T cu = up_self ? (ph == 0 ? T(vs.x[i]): T(vs.y[i]))
                                  : (ph == 0 ? T(vo.x[i]): T(vo.y[i]));
-            R.bal[i] -= cu * base;
+            if constexpr (MIRROR) {
+                const T cb = cu * base;
+                R.bal[i] -= cb;
+                mb[i] += cb;
+            } else {
+                R.bal[i] -= cu * base;
+            }
         }
     }
     if (!fick) return;
@@ -648,6 +667,14 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
             T Dt = MN * Dup;
             if (Cf == 0 && dcs == 0) Dt = T(0.0);
             R.bal[i] -= Dt * (xn - xs);
+            if constexpr (MIRROR) {
+                // the neighbour's row: upstream by its own comparison
+                T Dum = (xs > xn) ? T(ps.D[ph][i]): T(po.D[ph][i]);
+                if (zerophi) Dum = T(0.0);
+                T Dtm = MN * Dum;
+                if (Cf == 0 && dcs == 0) Dtm = T(0.0);
+                mb[i] += Dtm * (xn - xs);
+            }
         }
     }
 }
@@ -1547,10 +1574,11 @@ __global__ void k_reduce_expand(AsmArgs A, const double* S, const double* t, con
 //                derivatives along v_q / w, combined from the per-unknown
 //                ones (the property sweep is linear in the seed), and the
 //                direction's seeds of the block's variables;
-//   k_red_own / k_red_nb  the balance rows along direction q of the own
-//                block / of the blocks that depend on it (six face
-//                neighbours, the other continuum's block), written to the
-//                reduced block's column q, or (q = NP) to a per-slot partial;
+//   k_red_own    one lane per (block, direction q): the balance rows along
+//                direction q of the own block and of the blocks that depend
+//                on it (six face neighbours from the same face evaluations,
+//                the other continuum's block), written to the reduced
+//                blocks' column q, or (q = NP) to per-slot partials;
 //   k_red_rhs    one lane per block: -F_m plus the slot partials.
 
 // unknown column of variable m of a block in variable order (x_0..x_{nc-1},
@@ -1764,25 +1792,49 @@ __device__ __forceinline__ int dep_rank(const AsmArgs& A, int k, const Geo& G, i
     return rank;
 }
 
+// One lane per (block a, direction q): every reduced column that a's
+// direction q feeds.  The lane seeds a along q once (a's variables and
+// property derivatives in registers) and forms
+//   * a's own block: accumulation, local rows, its six faces and its
+//     transfer term;
+//   * the blocks of its face neighbours b, (b, a): each face's terms of b's
+//     balance rows, taken from the same face evaluation (face_piece MIRROR);
+//   * the other continuum's block of the same cell: its transfer term.
+// Each is written to column q of the reduced block, or (q = NP) to the
+// per-slot partial of the row block's right-hand side (slot 0 own, 1 + f
+// the row block's face f, 7 the transfer).  A zero-porosity block gets the
+// identity (Petsc_Functions.cpp:42051-42056) and still feeds its
+// neighbours; no row of a zero-porosity block is written here otherwise.
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, const double* S, const double* t,
-                                                 const int64_t* browptr, int32_t* bcols, double* bvals,
-                                                 double* rpart) {
+__global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, const int64_t* browptr,
+                                                 int32_t* bcols, double* bvals, double* rpart) {
     using D1 = DualV<1>;
-    constexpr int NLM = NC + 3, NP = NC + 1;
+    constexpr int NP = NC + 1;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= (int64_t)2 * A.n * (NP + 1)) return;
     const int q = (int)(item % (NP + 1));
     const int blk = (int)(item / (NP + 1));
     const int n = A.n, k = blk / n, c = blk % n;
+    const int64_t nb = 2 * (int64_t)n;
     const int64_t b0 = browptr[blk];
-    if (A.phi[k * n + c] < 1e-10) {
-        // identity block (Petsc_Functions.cpp:42051-42056); its right-hand side is 0
-        if (q == NP) return;
+    const bool live = A.phi[k * n + c] >= 1e-10;
+    if (!live && q < NP) {
         if (q == 0) bcols[b0] = blk;
         for (int i = 0; i < NP; i++) bvals[b0 * NP * NP + i * NP + q] = (i == q) ? 1.0 : 0.0;
-        return;
     }
+    // column q of reduced block (row block b, column block blk), or the
+    // slot partial of b's right-hand side
+    auto emit = [&](int b, int slot, int kb, const Geo& Gb, int cb, const double (&col)[NP]) {
+        if (q == NP) {
+#pragma unroll
+            for (int i = 0; i < NP; i++) rpart[((int64_t)slot * NP + i) * nb + b] = col[i];
+            return;
+        }
+        const int64_t bz = browptr[b] + dep_rank(A, kb, Gb, cb, blk);
+        if (q == 0) bcols[bz] = blk;
+#pragma unroll
+        for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = col[i];
+    };
     const Geo G = geom(A, c);
     VarsT<NC, double> v;
     load_vars<NC>(A, k, c, v);
@@ -1791,7 +1843,7 @@ __global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, 
     PropsT<NC, D1> sp;
     load_props_dual<NC>(A, dred, blk, q, sp, NP + 1, JacShape<NC>::RF);
     RowAcc<NC, D1> R;
-    own_piece<NC, D1>(A, k, c, G, sv, sp, R);
+    if (live) own_piece<NC, D1>(A, k, c, G, sv, sp, R);
     for (int f = 0; f < 6; f++) {
         const FaceGeo FG = face_geo(A, G, f);
         const int o = FG.o;
@@ -1800,89 +1852,42 @@ __global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, 
         PropsT<NC, double> po;
         load_vars<NC>(A, k, o, vo);
         load_props<NC>(A, k, o, po);
-        face_piece<NC, D1, double>(A, k, G, FG, f, c, o, sv, sp, vo, po, R);
+        D1 mb[NP];
+#pragma unroll
+        for (int i = 0; i < NP; i++) mb[i] = D1(0.0);
+        face_piece<NC, D1, double, true>(A, k, G, FG, f, c, o, sv, sp, vo, po, R, true, mb);
+        if (A.phi[k * n + o] >= 1e-10) {
+            double col[NP];
+#pragma unroll
+            for (int i = 0; i < NP; i++) col[i] = mb[i].d[0];
+            emit(k * n + o, (f ^ 1) + 1, k, geom(A, o), o, col);   // a sits across b's face f ^ 1
+        }
     }
     {
+        const int kx = 1 - k;
         VarsT<NC, double> vx;
         PropsT<NC, double> px;
-        load_vars<NC>(A, 1 - k, c, vx);
-        load_props<NC>(A, 1 - k, c, px);
-        if (k == 0) transfer_piece<NC, D1, double>(A, 0, G, c, sv, sp, vx, px, R);
-        else transfer_piece<NC, double, D1>(A, 1, G, c, vx, px, sv, sp, R);
-    }
-    if (q == NP) {
-        const int64_t nb = 2 * (int64_t)n;
-#pragma unroll
-        for (int i = 0; i < NP; i++) rpart[(int64_t)i * nb + blk] = R.bal[i].d[0];
-        return;
-    }
-    const int64_t bz = b0 + dep_rank(A, k, G, c, blk);
-    if (q == 0) bcols[bz] = blk;
-#pragma unroll
-    for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
-}
-
-// Neighbours, column-block-major: one lane per (column block d, direction
-// q).  The lane seeds d along q once -- d's variables and property
-// derivatives in registers -- and walks the row blocks that depend on d:
-// its six face neighbours b (d sits across the opposite face of b) and the
-// other continuum's block of the same cell (the transfer), writing column q
-// of each reduced block (b, d), or (q = NP) the per-slot partial of b.
-template <int NC>
-__global__ __launch_bounds__(256) void k_red_nb(AsmArgs A, const double* dred, const int64_t* browptr,
-                                                int32_t* bcols, double* bvals, double* rpart) {
-    using D1 = DualV<1>;
-    constexpr int NP = NC + 1;
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= (int64_t)2 * A.n * (NP + 1)) return;
-    const int q = (int)(item % (NP + 1));
-    const int d = (int)(item / (NP + 1));
-    const int n = A.n, k = d / n, cd = d % n;
-    const int64_t nb = 2 * (int64_t)n;
-    VarsT<NC, double> vd;
-    load_vars<NC>(A, k, cd, vd);
-    VarsT<NC, D1> sv;
-    seed_tab<NC>(vd, dred + (int64_t)d * JacShape<NC>::RF * (NP + 1) + q, NP + 1, A.pcoeff, sv);
-    PropsT<NC, D1> sp;
-    load_props_dual<NC>(A, dred, d, q, sp, NP + 1, JacShape<NC>::RF);
-    const Geo Gd = geom(A, cd);
-    auto emit = [&](int b, int slot, int kb, const Geo& G, int cb, const RowAcc<NC, D1>& R) {
-        if (q == NP) {
-#pragma unroll
-            for (int i = 0; i < NP; i++) rpart[((int64_t)slot * NP + i) * nb + b] = R.bal[i].d[0];
-            return;
+        load_vars<NC>(A, kx, c, vx);
+        load_props<NC>(A, kx, c, px);
+        if (live) {
+            if (k == 0) transfer_piece<NC, D1, double>(A, 0, G, c, sv, sp, vx, px, R);
+            else transfer_piece<NC, double, D1>(A, 1, G, c, vx, px, sv, sp, R);
         }
-        const int64_t bz = browptr[b] + dep_rank(A, kb, G, cb, d);
-        if (q == 0) bcols[bz] = d;
+        if (A.phi[kx * n + c] >= 1e-10) {
+            RowAcc<NC, D1> Rx;
+            if (k == 0) transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, vx, px, Rx);
+            else transfer_piece<NC, double, D1>(A, 0, G, c, vx, px, sv, sp, Rx);
+            double col[NP];
 #pragma unroll
-        for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = R.bal[i].d[0];
-    };
-#pragma unroll 1
-    for (int fp = 0; fp < 6; fp++) {
-        const int cb = nb_of(A, Gd, fp);
-        if (cb < 0 || A.phi[k * n + cb] < 1e-10) continue;   // no neighbour / identity row block
-        const int f = fp ^ 1;                                // d as seen from b
-        const Geo G = geom(A, cb);
-        VarsT<NC, double> v;
-        load_vars<NC>(A, k, cb, v);
-        PropsT<NC, double> p;
-        load_props<NC>(A, k, cb, p);
-        RowAcc<NC, D1> R;
-        const FaceGeo FG = face_geo(A, G, f);
-        face_piece<NC, double, D1>(A, k, G, FG, f, cb, cd, v, p, sv, sp, R);
-        emit(k * n + cb, f + 1, k, G, cb, R);
+            for (int i = 0; i < NP; i++) col[i] = Rx.bal[i].d[0];
+            emit(kx * n + c, 7, kx, G, c, col);
+        }
     }
-    const int kb = 1 - k;
-    if (A.phi[kb * n + cd] >= 1e-10) {
-        VarsT<NC, double> v;
-        load_vars<NC>(A, kb, cd, v);
-        PropsT<NC, double> p;
-        load_props<NC>(A, kb, cd, p);
-        RowAcc<NC, D1> R;
-        if (kb == 0) transfer_piece<NC, double, D1>(A, 0, Gd, cd, v, p, sv, sp, R);
-        else transfer_piece<NC, D1, double>(A, 1, Gd, cd, sv, sp, v, p, R);
-        emit(kb * n + cd, 7, kb, Gd, cd, R);
-    }
+    if (!live) return;
+    double col[NP];
+#pragma unroll
+    for (int i = 0; i < NP; i++) col[i] = R.bal[i].d[0];
+    emit(blk, 0, k, G, c, col);
 }
 
 template <int NC>
@@ -2106,9 +2111,7 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
                        0, st, A, (const double*)a->wloc, F, (const int64_t*)nullptr, (const double*)nullptr, S, t);
     hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)S, (const double*)t,
                        a->dred);
-    hipLaunchKernelGGL(osr::k_red_own<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, (const double*)S,
-                       (const double*)t, browptr, bcols, bvals, a->rpart);
-    hipLaunchKernelGGL(osr::k_red_nb<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
+    hipLaunchKernelGGL(osr::k_red_own<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
                        bvals, a->rpart);
     hipLaunchKernelGGL(osr::k_red_rhs<NC>, dim3(g1), dim3(256), 0, st, A, F, (const double*)a->rpart, brhs);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
