@@ -26,6 +26,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "dual_dev.hpp"
 #include "osr_internal.hpp"
@@ -1391,42 +1392,59 @@ __device__ __forceinline__ double csr_at(const AsmArgs& A, int blk, int lab, int
 // t = J_ls^-1 F_l.  nl and lab are uniform over the group.
 constexpr int RED_GROUP = 32;
 
-template <int NC>
-__device__ __forceinline__ void group_eliminate(double (&col)[NC + 3], int lab, int nl, int gbase) {
-    constexpr int NLM = NC + 3;
-#pragma unroll
-    for (int q = 0; q < NLM; q++) {
-        if (q >= nl) break;
+// Calls f(std::integral_constant<int, B>) ... f(<E - 1>): straight-line code
+// with every array index a constant from the start, so per-lane arrays stay
+// in registers (a loop whose body selects an element by a run-time index,
+// such as a pivot-row swap, is otherwise turned into an indexed access of a
+// stack array).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// NL = the block's number of local rows, a compile-time shape: nc + 3 for a
+// two-phase block (lab = 0), 2 for a single-phase one.
+template <int NC, int NL>
+__device__ __forceinline__ void group_eliminate(double (&col)[NC + 3], int lab, int gbase) {
+    if (NL == NC + 3) lab = 0;
+    static_for<0, NL>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
         const int P = gbase + red_sec_col(lab, NC, q);
         int piv = q;
         double best = fabs(col[q]);
-#pragma unroll
-        for (int r = q + 1; r < NLM; r++) {
+        static_for<q + 1, NL>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
             const double v = fabs(col[r]);
-            if (r < nl && v > best) {
+            if (v > best) {
                 best = v;
                 piv = r;
             }
-        }
+        });
         piv = __shfl(piv, P, 64);
         const double cq = col[q];
         double cp = cq;
-#pragma unroll
-        for (int r = q + 1; r < NLM; r++)
-            if (r == piv) cp = col[r];
-#pragma unroll
-        for (int r = q + 1; r < NLM; r++)
-            if (r == piv) col[r] = cq;
+        static_for<q + 1, NL>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            const bool m = r == piv;
+            cp = m ? col[r] : cp;
+            col[r] = m ? cq : col[r];
+        });
         col[q] = cp;
-        double f[NLM];
-#pragma unroll
-        for (int r = 0; r < NLM; r++) f[r] = __shfl(col[r], P, 64);
+        double f[NL];
+        static_for<0, NL>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            f[r] = __shfl(col[r], P, 64);
+        });
         const double cn = col[q] * (1.0 / f[q]);
-#pragma unroll
-        for (int r = 0; r < NLM; r++)
-            if (r != q && r < nl) col[r] -= f[r] * cn;
+        static_for<0, NL>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if (r != q) col[r] -= f[r] * cn;
+        });
         col[q] = cn;
-    }
+    });
 }
 
 // S, t of one block from the group's columns (zero beyond the block's nl rows)
@@ -1708,7 +1726,9 @@ __global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, 
                 if (r < nl) col[r] = w[r * JacShape<NC>::MAXC];
         }
     }
-    group_eliminate<NC>(col, lab, nl, gbase);
+    // a group is one block, so each group runs one static shape
+    if (nl == NLM) group_eliminate<NC, NC + 3>(col, lab, gbase);
+    else group_eliminate<NC, 2>(col, lab, gbase);
     group_write_st<NC>(col, lab, nl, j, bs, Sb, tb);
 }
 
