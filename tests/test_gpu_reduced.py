@@ -51,6 +51,16 @@ def _cases(oracle):
     grid.phi_m[[0, 5]] = 0.0
     grid.phi_f[[3]] = 0.0
     yield "c0_zero_porosity", (fl, grid, st)
+    # every cell with one two-phase cell's compositions (pressures still
+    # differ, so the diffusion coefficients differ): at every face x_s = x_o,
+    # where each side's Fickian term takes its own coefficient (the tie of
+    # the upstream rule), so the neighbour's row is not the mirror image of
+    # the own row there
+    grid, st = synth.dual_porosity_case(fl6, 5, 4, 3, 4, oracle_flash_fn(oracle, fl6), p_range=(8e6, 20e6))
+    c2 = int(np.flatnonzero(st.label_m == 0)[0])
+    xr, yr = st.x_m[:, [c2]].copy(), st.y_m[:, [c2]].copy()
+    st.x_m[:], st.x_f[:], st.y_m[:], st.y_f[:] = xr, xr, yr, yr
+    yield "pr6_equal_compositions", (fl6, grid, st)
 
 
 def _check_identity_blocks(grid, np1, red):
@@ -133,6 +143,9 @@ def test_reduced_system_solves_the_full_one(gpu, oracle):
         red_direct = asm.reduced_direct(d, F)
         torch.cuda.synchronize()
         nan_state = name in ("c0_zero_porosity", "pr6_liquid_gas")
+        if name == "pr6_equal_compositions":
+            D = asm.props(d).cpu().numpy()[16:16 + 2 * fl.nc]   # [field][block]
+            assert np.ptp(D[:, :grid.n], axis=1).min() > 0, "diffusion coefficients must differ between cells"
         print(name, "direct vs CSR reduction, worst row-scaled difference",
               _same_reduction(name, red, red_direct, np1, nan_rows=nan_state), flush=True)
         if name == "pr6_liquid_gas":
