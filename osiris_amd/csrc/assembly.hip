@@ -336,40 +336,6 @@ __device__ __forceinline__ void propsT(const AsmArgs& A, int k, int c, int lab, 
     p.gam[1] = G_ACCG * p.ro[1] * mg;
 }
 
-// The part of the property sweep the block-local rows read: Po, Pg and the
-// two phase densities (and the pore-volume factor), by the same expressions
-// as propsT.
-template <int NC, class T>
-__device__ __forceinline__ void propsT_loc(const AsmArgs& A, int k, int c, int lab, const T& Po, const T* x,
-                                           const T* y, const T* S, double rpo, double rpg, PropsT<NC, T>& p,
-                                           double zl = 0.0, double zg = 0.0) {
-    const AsmConsts& K = *A.K;
-    const double phi = A.phi[k * A.n + c];
-    const T dP = Po - K.refP;
-    p.Po = Po;
-    p.porpor = phi * (1.0 + K.cpor * dP + K.dcpor * dP * dP);
-    if (k == 0) {
-        if (lab == 0) {
-            T ps = 0.0;
-#pragma unroll
-            for (int i = 0; i < NC; i++) ps += K.par[i] * (x[i] * rpo - y[i] * rpg);
-            p.bift = fpow(ps * 1e-6, 3.88);
-        } else {
-            p.bift = T(IFTBASECASE);
-        }
-        T krg, krog, pcg;
-        table<T>(K.sgt, K.nsgt, S[2], krg, krog, pcg);
-        if (p.bift < MISCIBLEIFT) p.Pg = Po + pcg * p.bift / IFTBASECASE;
-        else p.Pg = Po + pcg;
-    } else {
-        p.bift = T(IFTBASECASE);
-        p.Pg = Po;
-    }
-    const double rt = RGAS * K.T;
-    p.ro[0] = p.Po / (zphase<NC, T>(K, x, p.Po, false, zl) * rt);
-    p.ro[1] = p.Pg / (zphase<NC, T>(K, y, p.Pg, true, zg) * rt);
-}
-
 // the two phase roots of block blk from the property table (Z = P / (rho R T))
 __device__ __forceinline__ void table_roots(const AsmArgs& A, int blk, double& zl, double& zg) {
     const int64_t s2 = 2 * (int64_t)A.n;
@@ -1635,10 +1601,98 @@ __device__ __forceinline__ void seed_tab(const VarsT<NC, double>& v, const doubl
     for (int j = 0; j < 3; j++) out.S[j] = D(v.S[j], 2 * NC + 1 + j, 1.0);
 }
 
+// The local rows of a block (fugacity and constraint rows) as the Po, Pg and
+// density part of propsT and then loc_piece compute them, phase by phase:
+// each phase's mixing sums serve
+// both its density (the root's derivative) and its ln phi, and only one
+// phase's temporaries are live at a time.  Values of Po, Pg and the
+// densities are the property table's (tab, stride s2), derivatives from the
+// forward-mode sweep.
+template <int NC>
+__device__ __forceinline__ void red_loc_rows(const AsmArgs& A, int k, int lab, const VarsT<NC, DualV<1>>& v,
+                                             double rpo, double rpg, double zl, double zg, const double* tab,
+                                             int64_t s2, RowAcc<NC, DualV<1>>& R) {
+    using T = DualV<1>;
+    const AsmConsts& K = *A.K;
+    const double rt = RGAS * K.T;
+    if (lab != 0) {
+        T sm = 0.0;
+#pragma unroll
+        for (int i = 0; i < NC; i++) sm += (lab == 1 ? v.x[i] : v.y[i]);
+        R.loc[0] += sm - 1.0;
+        R.loc[1] += v.S[0] + v.S[1] + v.S[2] - 1.0;
+        return;
+    }
+    // Pg as propsT forms it (IFT-scaled gas capillary pressure in the matrix)
+    T Po = v.Po, Pg;
+    if (k == 0) {
+        T ps = 0.0;
+#pragma unroll
+        for (int i = 0; i < NC; i++) ps += K.par[i] * (v.x[i] * rpo - v.y[i] * rpg);
+        const T bift = fpow(ps * 1e-6, 3.88);
+        T krg, krog, pcg;
+        table<T>(K.sgt, K.nsgt, v.S[2], krg, krog, pcg);
+        if (bift < MISCIBLEIFT) Pg = Po + pcg * bift / IFTBASECASE;
+        else Pg = Po + pcg;
+    } else {
+        Pg = Po;
+    }
+    const T Pg_sweep = Pg;   // zphase's pressure: the sweep's own value
+    Po.v = tab[2 * s2];
+    Pg.v = tab[3 * s2];
+    T fL[NC];
+#pragma unroll
+    for (int ph = 0; ph < 2; ph++) {
+        const T* xp = ph == 0 ? v.x : v.y;
+        const T& P = ph == 0 ? Po : Pg;
+        T Ca, Cb, t[NC];
+        mixT<NC, T>(K, xp, Ca, Cb, t);
+        // density: the root's derivative from the implicit function (the
+        // value from the table), as zphase
+        T ro;
+        {
+            // zphase forms A, B from the sweep's own pressure (its value the state's)
+            const T Pz = ph == 0 ? v.Po : Pg_sweep;
+            const T Az = Ca * Pz / (rt * rt);
+            const T Bz = Cb * Pz / rt;
+            const bool srk = K.eos == OSR_EOS_SRK;
+            const T U = srk ? Bz : 2.0 * Bz;
+            const T W = srk ? T(0.0) : Bz;
+            const T Z = zroot_k(-(1.0 + Bz - U), Az - Bz * U - U - W * W, -(Az * Bz - Bz * W * W - W * W), ph == 1,
+                                ph == 0 ? zl : zg);
+            ro = Pz / (Z * rt);
+            ro.v = tab[(5 + ph) * s2];
+        }
+        const T Ap = Ca * P / (rt * rt), Bp = Cb * P / rt;
+        const T Zp = P / (ro * rt);
+        const T DD = Ap / (Bp * (K.d1 - K.d2));
+        const T FF = flog((Zp + K.d2 * Bp) / (Zp + K.d1 * Bp));
+        const T lz = flog(Zp - Bp);
+        const T iCa = 1.0 / Ca, iCb = 1.0 / Cb;
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            const T t1 = K.b[i] * iCb;
+            const T EE = 2.0 * t[i] * iCa - t1;
+            const T phi = fexp(t1 * (Zp - 1.0) - lz + DD * EE * FF);
+            if (ph == 0) fL[i] = xp[i] * P * phi;
+            else R.loc[i] += fL[i] - xp[i] * P * phi;
+        }
+    }
+    T sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        sx += v.x[i];
+        sy += v.y[i];
+    }
+    R.loc[NC] += sx - 1.0;
+    R.loc[NC + 1] += sy - 1.0;
+    R.loc[NC + 2] += v.S[0] + v.S[1] + v.S[2] - 1.0;
+}
+
 // one lane per (block, unknown j): column j of the block's local rows
 // (fugacity and constraint rows) by forward mode with unknown j seeded --
 // they read only Po, Pg and the phase densities, so only that part of the
-// property sweep runs (propsT_loc), with the values of the property table
+// property sweep runs (red_loc_rows), with the values of the property table
 // and its phase roots -- into Wl laid out [block][local row][column]
 template <int NC>
 __global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, double* Wl) {
@@ -1663,16 +1717,9 @@ __global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, double* Wl) {
     }
     double zl, zg;
     table_roots(A, blk, zl, zg);
-    PropsT<NC, D1> sp;
-    propsT_loc<NC, D1>(A, k, c, lab, sv.Po, sv.x, sv.y, sv.S, rpo, rpg, sp, zl, zg);
     const int64_t s2 = 2 * (int64_t)n;
-    const double* tab = A.props + blk;
-    sp.Po.v = tab[2 * s2];
-    sp.Pg.v = tab[3 * s2];
-    sp.ro[0].v = tab[5 * s2];
-    sp.ro[1].v = tab[6 * s2];
     RowAcc<NC, D1> R;
-    loc_piece<NC, D1>(A, sv, sp, R);
+    red_loc_rows<NC>(A, k, lab, sv, rpo, rpg, zl, zg, A.props + blk, s2, R);
     const int nl = nloc(lab, NC);
     double* w = Wl + (int64_t)blk * MAXC * (NC + 3) + j;
 #pragma unroll
