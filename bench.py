@@ -283,13 +283,24 @@ def newton_leg(rt, synth, torch, args, log):
     torch.cuda.empty_cache()
     log(f"newton: plain {ms_plain:.1f} ms on the big grid")
 
-    # analytic path with the mixed FP32/FP64 flash on the same grid
+    # analytic path with the mixed FP32/FP64 flash on the same grid, its
+    # launch knobs chosen by the autotuner on this grid's cells
+    from osiris_amd import tune
+    gb, sb = big
+    Pb = torch.tensor(np.concatenate([sb.Po_m, sb.Po_f]), device="cuda")
+    zb = torch.tensor(np.concatenate([sb.z_m, sb.z_f], axis=1), device="cuda")
+    tuned = tune.tune_flash(ctx, gb.T, Pb, zb, mode="mixed", reps=3)
+    tune.apply(ctx, tuned)
+    rt.flash(ctx, gb.T, Pb, zb, mode="mixed")
+    mixed_stats = rt.flash_stats(ctx)
+    del Pb, zb
     b = NewtonBatch(ctx, fl, [big], form="reduced", flash_mode="mixed")
     ms_mixedflash_big, _ = time_graph(b, torch, steps_big, 1)
     b.close()
     del b
+    tune.apply(ctx, {"config": tune.DEFAULTS["persistent"]})
     torch.cuda.empty_cache()
-    log(f"newton: analytic + mixed flash {ms_mixedflash_big:.1f} ms on the big grid")
+    log(f"newton: analytic + mixed flash {ms_mixedflash_big:.1f} ms on the big grid ({tuned['config']})")
 
     # analytic path with the persistent FP64 flash, plus the reduced assembly alone
     b = NewtonBatch(ctx, fl, [big], form="reduced", flash_mode=FAST_FLASH)
@@ -444,7 +455,9 @@ def newton_leg(rt, synth, torch, args, log):
                                      "reduced_blocks": nnzb_big, "flash": FAST_FLASH},
                 "analytic_reduced_mixed_flash": {"ms_per_iter": ms_mixedflash_big,
                                                  "cells_per_s": nbig / (ms_mixedflash_big * 1e-3),
-                                                 "flash": "mixed FP32 successive substitution + FP64 Newton"},
+                                                 "flash": "mixed FP32 successive substitution + FP64 Newton",
+                                                 "tuned_knobs": tuned["config"],
+                                                 "flash_stages": {k: int(v) for k, v in mixed_stats.items()}},
                 "speedup_analytic_vs_plain": ms_plain / ms_mixed_big,
                 "reduced_assembly_ms": ms_red,
                 "linear_solve": {
@@ -710,16 +723,22 @@ def cpu_newton_sample(rt, synth, torch):
     T = np.full(2 * n, grid.T)
     P = np.concatenate([st.Po_m, st.Po_f])
     z = np.concatenate([st.z_m, st.z_f], axis=1)
-    t0 = time.perf_counter()
+    tp = [time.perf_counter()]
     O.flash_batch(fl, T, P, z, nthreads=1)
+    tp.append(time.perf_counter())
     pr = O.props(fl, grid, st)
+    tp.append(time.perf_counter())
     for k, (lab, Po, x, y, S) in enumerate(((st.label_m, st.Po_m, st.x_m, st.y_m, st.S_m),
                                             (st.label_f, st.Po_f, st.x_f, st.y_f, st.S_f))):
         O.phase_ctrl(fl, np.full(n, grid.T), lab.copy(), Po, pr[k, :, 3].copy(), x.copy(), y.copy(), S.copy(),
                      nthreads=1)
+    tp.append(time.perf_counter())
     F_or = O.residual(fl, grid, st)
+    tp.append(time.perf_counter())
     rp, cols, vals = O.jacobian(fl, grid, st)
-    cpu_s = time.perf_counter() - t0
+    tp.append(time.perf_counter())
+    cpu_s = tp[-1] - tp[0]
+    phases = dict(zip(("flash", "property_sweep", "phase_test", "residual", "jacobian"), np.diff(tp).tolist()))
     # the GPU on the same sample (CSR form), against the oracle
     ctx = rt.Context(fl)
     b = NewtonBatch(ctx, fl, [(grid, st)], form="csr", flash_mode="plain")
@@ -746,7 +765,7 @@ def cpu_newton_sample(rt, synth, torch):
     return {"value": 2 * n / cpu_s, "unit": "cells/s", "cores": 1, "kind": "port",
             "sample": f"16x16x8 dual-porosity grid of the same workload ({2 * n} cells, 10 components): oracle "
                       "flash of every cell + PhaseCtrl + residual + reference-layout Jacobian, one thread",
-            "seconds": cpu_s}, {"jacobian_pattern_identical": same_pattern,
+            "seconds": cpu_s, "phase_seconds": phases}, {"jacobian_pattern_identical": same_pattern,
                                 "residual_max_row_scaled_err": f_err,
                                 "jacobian_max_rowwise_rel_err": j_row,
                                 "jacobian_max_rel_err_entries_above_1e-4_of_row": j_entry,

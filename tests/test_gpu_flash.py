@@ -195,6 +195,33 @@ def test_mixed_precision_stages(gpu, oracle, nc, eos, seed):
     assert st["parked"] >= two and st["newton_ok"] > 0.8 * two, (st, two)
 
 
+@pytest.mark.parametrize("knobs", [
+    # the set the autotuner picks on the 10-component dual-porosity Newton
+    # workload (98 % two-phase): long FP64 warm-up, tight FP32 stop
+    {"flash.mixed_warm_iters": 40, "flash.mixed_tol32": 3e-5, "flash.mixed_max32": 40},
+    {"flash.mixed_warm_iters": 16, "flash.mixed_tol32": 1e-5, "flash.mixed_max32": 250},
+])
+def test_mixed_precision_tuned_knobs(gpu, oracle, knobs):
+    """Mixed path under other launch knobs, on cells like the Newton
+    workload's (10 components PR, T = 360 K, p ~ U(8, 20) MPa, z ~
+    Dirichlet(2)): the same parity bar; the knobs move cells between the
+    stages, not the answer."""
+    fl = synth.fluid_pseudo(10, synth.PR, 11)
+    g = synth.rng(21)
+    n = 8000
+    P = g.uniform(8e6, 20e6, n)
+    z = g.dirichlet(np.full(10, 2.0), n).T.copy()
+    ref = oracle.flash_batch(fl, 360.0, P, z)
+    ctx = _ctx(gpu, fl)
+    for k, v in knobs.items():
+        ctx.tune(k, v)
+    out = gpu.flash(ctx, np.full(n, 360.0), P, z, mode="mixed")
+    st = gpu.flash_stats(ctx)
+    print(knobs, st)
+    compare_flash(out, ref, what=f"mixed {knobs}", tol_capped=TOL_CAPPED_MIXED)
+    assert st["newton"] > 0 and st["newton_ok"] > 0.8 * st["newton"], st
+
+
 def test_flash_edge_cases(gpu, oracle):
     fl = synth.fluid_pseudo(8, synth.PR, 1)
     ctx = _ctx(gpu, fl)
