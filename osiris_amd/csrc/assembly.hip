@@ -541,7 +541,7 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
                                            int o, const VarsT<NC, TS>& vs, const PropsT<NC, TS>& ps,
                                            const VarsT<NC, TO>& vo, const PropsT<NC, TO>& po,
                                            RowAcc<NC, decltype(TS() + TO())>& R, bool fick = true,
-                                           decltype(TS() + TO())* mb = nullptr) {
+                                           decltype(TS() + TO())* mb = nullptr, bool conv = true) {
     using T = decltype(TS() + TO());
     const AsmConsts& K = *A.K;
     const int d = q >> 1;
@@ -554,7 +554,7 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
     const double hb = self_is_a ? FG.hn : G.h;
     const double dh = FG.hn - G.h;
     // water (upstream by water potential)
-    {
+    if (conv) {
         const T pot = (self_is_a ? (ps.Pw - po.Pw) : (po.Pw - ps.Pw)) + G_ACC * K.wat_ro * (ha - hb);
         const bool up_a = pot > 0;
         const bool up_self = (up_a == self_is_a);
@@ -566,6 +566,7 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
     // oil and gas
 #pragma unroll
     for (int ph = 0; ph < 2; ph++) {
+        if (!conv) break;
         const int absent = ph == 0 ? -1 : 1;
         const double cs = vs.lab == absent ? 0.0 : 1.0;
         const double co = vo.lab == absent ? 0.0 : 1.0;
@@ -1918,11 +1919,33 @@ __global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, 
         VarsT<NC, double> vo;
         PropsT<NC, double> po;
         load_vars<NC>(A, k, o, vo);
-        load_props<NC>(A, k, o, po);
+        {
+            // the fields of the convective and gravity terms only
+            const int64_t s2 = 2 * (int64_t)n;
+            const double* qn = A.props + (int64_t)k * n + o;
+            po.Pw = qn[1 * s2]; po.Po = qn[2 * s2]; po.Pg = qn[3 * s2];
+            po.ro[0] = qn[5 * s2]; po.ro[1] = qn[6 * s2]; po.mu[0] = qn[7 * s2]; po.mu[1] = qn[8 * s2];
+            po.gam[0] = qn[9 * s2]; po.gam[1] = qn[10 * s2];
+            po.kr[0] = qn[11 * s2]; po.kr[1] = qn[12 * s2]; po.kr[2] = qn[13 * s2];
+        }
         D1 mb[NP];
 #pragma unroll
         for (int i = 0; i < NP; i++) mb[i] = D1(0.0);
-        face_piece<NC, D1, double, true>(A, k, G, FG, f, c, o, sv, sp, vo, po, R, true, mb);
+        face_piece<NC, D1, double, true>(A, k, G, FG, f, c, o, sv, sp, vo, po, R, false, mb, true);
+        // the Fickian terms' fields are loaded only now, so they are not
+        // live across the convective part
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            const int64_t s2 = 2 * (int64_t)n;
+            const double* qn = A.props + (int64_t)k * n + o;
+            po.porpor = qn[0];
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+                po.D[0][i] = qn[(16 + i) * s2];
+                po.D[1][i] = qn[(16 + NC + i) * s2];
+            }
+        }
+        face_piece<NC, D1, double, true>(A, k, G, FG, f, c, o, sv, sp, vo, po, R, true, mb, false);
         if (A.phi[k * n + o] >= 1e-10) {
             double col[NP];
 #pragma unroll
