@@ -293,12 +293,15 @@ def newton_leg(rt, synth, torch, args, log):
     tune.apply(ctx, tuned)
     rt.flash(ctx, gb.T, Pb, zb, mode="mixed")
     mixed_stats = rt.flash_stats(ctx)
-    del Pb, zb
     b = NewtonBatch(ctx, fl, [big], form="reduced", flash_mode="mixed")
     ms_mixedflash_big, _ = time_graph(b, torch, steps_big, 1)
     b.close()
     del b
-    tune.apply(ctx, {"config": tune.DEFAULTS["persistent"]})
+    # the persistent FP64 flash of the analytic path: its knobs tuned on the
+    # same cells (kept for every later persistent run of this leg)
+    tuned_p = tune.tune_flash(ctx, gb.T, Pb, zb, mode="persistent", reps=3)
+    tune.apply(ctx, tuned_p)
+    del Pb, zb
     torch.cuda.empty_cache()
     log(f"newton: analytic + mixed flash {ms_mixedflash_big:.1f} ms on the big grid ({tuned['config']})")
 
@@ -452,7 +455,8 @@ def newton_leg(rt, synth, torch, args, log):
                 "plain_fp64_csr": {"ms_per_iter": ms_plain, "cells_per_s": nbig / (ms_plain * 1e-3),
                                    "jacobian_nnz": nnz_plain},
                 "analytic_reduced": {"ms_per_iter": ms_mixed_big, "cells_per_s": nbig / (ms_mixed_big * 1e-3),
-                                     "reduced_blocks": nnzb_big, "flash": FAST_FLASH},
+                                     "reduced_blocks": nnzb_big, "flash": FAST_FLASH,
+                                     "tuned_knobs": tuned_p["config"]},
                 "analytic_reduced_mixed_flash": {"ms_per_iter": ms_mixedflash_big,
                                                  "cells_per_s": nbig / (ms_mixedflash_big * 1e-3),
                                                  "flash": "mixed FP32 successive substitution + FP64 Newton",

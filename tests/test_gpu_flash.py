@@ -222,6 +222,31 @@ def test_mixed_precision_tuned_knobs(gpu, oracle, knobs):
     assert st["newton"] > 0 and st["newton_ok"] > 0.8 * st["newton"], st
 
 
+def test_persistent_tuned_knobs(gpu, oracle):
+    """Persistent flash under the launch knobs the autotuner picks on the
+    Newton workload (refill at 16 idle lanes, 3 blocks per CU, hand-off to
+    the lane-group kernel after 100 passes), on cells like that workload's:
+    the same parity bar, and bitwise equal to the default knobs on every
+    cell the hand-off does not touch."""
+    fl = synth.fluid_pseudo(10, synth.PR, 11)
+    g = synth.rng(22)
+    n = 8000
+    P = g.uniform(8e6, 20e6, n)
+    z = g.dirichlet(np.full(10, 2.0), n).T.copy()
+    T = np.full(n, 360.0)
+    ref = oracle.flash_batch(fl, 360.0, P, z)
+    ctx = _ctx(gpu, fl)
+    base = {k: v.cpu().numpy() for k, v in gpu.flash(ctx, T, P, z, mode="persistent").items()}
+    for k, v in {"flash.refill_min": 16, "flash.blocks_per_cu": 3, "flash.park_after": 100}.items():
+        ctx.tune(k, v)
+    out = gpu.flash(ctx, T, P, z, mode="persistent")
+    compare_flash(out, ref, what="persistent tuned knobs")
+    early = base["iters"] < 40           # never handed off under either setting
+    for k in ("x", "y", "K", "L", "label", "iters"):
+        a, b = out[k].cpu().numpy(), base[k]
+        assert np.array_equal(a[..., early], b[..., early], equal_nan=k in ("x", "y", "K", "L")), k
+
+
 def test_flash_edge_cases(gpu, oracle):
     fl = synth.fluid_pseudo(8, synth.PR, 1)
     ctx = _ctx(gpu, fl)
