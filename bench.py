@@ -544,6 +544,9 @@ def flash_leg(rt, synth, torch, args):
         kern = float(np.mean([s_.elapsed_time(e) for s_, e in ev])) * 1e-3
         return wall / steps, kern
 
+    from osiris_amd import tune
+    tuned = {"persistent": tune.tune_flash(ctx, T, P, z, mode="persistent", reps=3)["config"]}
+    tune.apply(ctx, {"config": tuned["persistent"]})
     out = rt.alloc_flash_out(NC, n)
     sec_persist, kern_persist = run("persistent", out, steps, 1)
     iters = out["iters"].cpu().numpy()
@@ -552,6 +555,8 @@ def flash_leg(rt, synth, torch, args):
     gpu_res = {k: out[k].cpu().numpy() for k in ("x", "y", "K", "L", "label", "iters")}
     out_plain = rt.alloc_flash_out(NC, n)
     sec_plain, kern_plain = run("plain", out_plain, max(3, steps // 4), 1)
+    tuned["mixed"] = tune.tune_flash(ctx, T, P, z, mode="mixed", reps=3)["config"]
+    tune.apply(ctx, {"config": tuned["mixed"]})
     out_mixed = rt.alloc_flash_out(NC, n)
     sec_mixed, kern_mixed = run("mixed", out_mixed, steps, 1)
     mixed_res = {k: out_mixed[k].cpu().numpy() for k in ("x", "y", "K", "L", "label", "iters")}
@@ -567,6 +572,7 @@ def flash_leg(rt, synth, torch, args):
         "mixed_fp32_fp64": {"cells_per_s": n / sec_mixed, "kernel_ms": kern_mixed * 1e3},
         "speedup_persistent_vs_plain": sec_plain / sec_persist,
         "speedup_mixed_vs_plain": sec_plain / sec_mixed,
+        "tuned_knobs": tuned,
         "workload_stats": {
             "mean_ss_passes": float(iters.mean()), "max_ss_passes": int(iters.max()),
             "frac_two_phase": float((labels == 0).mean()),
@@ -623,7 +629,13 @@ def srk_mixed_leg(rt, synth, torch, args):
     keys = ("x", "y", "K", "L", "label", "iters")
     res = {}
     ms = {}
+    from osiris_amd import tune
+    tuned = {}
     for mode, k in (("plain", max(3, steps // 4)), ("persistent", steps), ("mixed", steps)):
+        if mode != "plain":
+            # launch knobs of this form chosen by the autotuner on these cells
+            tuned[mode] = tune.tune_flash(ctx, T, P, z, mode=mode, reps=3)["config"]
+            tune.apply(ctx, {"config": tuned[mode]})
         out = rt.alloc_flash_out(SRK_NC, n)
         ms[mode] = run(mode, out, k)
         if mode == "mixed":
@@ -656,6 +668,7 @@ def srk_mixed_leg(rt, synth, torch, args):
                             "stages": ms["mixed_stats"]},
         "speedup_mixed_vs_plain": ms["plain"] / ms["mixed"],
         "speedup_persistent_vs_plain": ms["plain"] / ms["persistent"],
+        "tuned_knobs": tuned,
         "frac_two_phase": float((b["label"] == 0).mean()),
         "mixed_vs_gpu_fp64_every_cell": vs_fp64,
     }
