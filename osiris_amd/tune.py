@@ -34,8 +34,8 @@ DEFAULTS = {
 
 SPACE = {
     "flash.refill_min": [8, 16, 32, 48, 64],
-    "flash.park_after": [0, 16, 24, 40, 64, 100],
-    "flash.blocks_per_cu": [0, 1, 2, 3],
+    "flash.park_after": [0, 16, 24, 40, 64, 100, 160, 250],
+    "flash.blocks_per_cu": [0, 1, 2, 3, 4, 6],
     "flash.mixed_tol32": [1e-4, 3e-5, 1e-5, 3e-6],
     "flash.mixed_max32": [40, 80, 150, 250],
     "flash.mixed_warm_iters": [2, 3, 5, 8, 16, 24, 40],
