@@ -1783,9 +1783,11 @@ __global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, 
 // one lane per (block, direction q <= NP): the property sweep in forward
 // mode along the reduced direction q of the block (v_q, or w for q = NP;
 // the phase roots from the property table), written with the direction's
-// seeds of the block's variables
+// seeds of the block's variables.  Built for two waves per SIMD (registers
+// capped at 256 per lane, the rest spilled): the latency of the sweep's
+// chains hides better than at one wave with no spill.
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_dprops(AsmArgs A, const double* S, const double* t, double* dred) {
+__global__ __launch_bounds__(256, 2) void k_red_dprops(AsmArgs A, const double* S, const double* t, double* dred) {
     using D1 = DualV<1>;
     constexpr int NF = JacShape<NC>::NF, NLM = NC + 3, NP = NC + 1;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
