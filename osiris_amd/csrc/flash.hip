@@ -568,8 +568,11 @@ __global__ __launch_bounds__(256) void k_mixed_ss32(const FluidDev* __restrict__
 constexpr int MIXED_NEWTON_MAX = 8;
 constexpr double MIXED_ACCEPT = 1e-10;   // max |ln K'/K| of the final reference pass
 
+// Built for two waves per SIMD (registers capped at 256 per lane; a few
+// spill at nc >= 10): the polish's cross-lane chains hide better with the
+// second wave.
 template <int NC, int S>
-__global__ __launch_bounds__(256) void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
+__global__ __launch_bounds__(256, 2) void k_mixed_newton(const FluidDev* __restrict__ F, FlashArgs a) {
 #pragma clang fp contract(off)
     constexpr int G = 2 * S;
     constexpr int GPW = 64 / G;
