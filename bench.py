@@ -21,7 +21,9 @@ parity against the oracle), the mixed-precision configuration (12
 pseudo-components, SRK, 4M independent cells; plain, persistent and mixed
 forms, K-values, phase fractions and ln-fugacities per cell against the
 oracle) and the 64x64x8 differentiation configuration (permeability
-adjoint, flash adjoint).  The CPU baseline is the oracle (FP64
+adjoint, flash adjoint).  Each persistent or mixed flash runs under the
+launch knobs the autotuner (osiris_amd/tune.py) picks on that workload's own
+cells; the knobs are reported in the line.  The CPU baseline is the oracle (FP64
 C/C++ restatement) on bounded samples of the same workloads.
 
 Prints ONE JSON line (rank 0).  Only one GPU is used; --gpus > 1 exits.
