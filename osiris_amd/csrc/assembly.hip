@@ -1696,7 +1696,7 @@ __device__ __forceinline__ void red_loc_rows(const AsmArgs& A, int k, int lab, c
 // property sweep runs (red_loc_rows), with the values of the property table
 // and its phase roots -- into Wl laid out [block][local row][column]
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_loc(AsmArgs A, double* Wl) {
+__global__ __launch_bounds__(64) void k_red_loc(AsmArgs A, double* Wl) {
     using D1 = DualV<1>;
     constexpr int MAXC = JacShape<NC>::MAXC;
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1876,7 +1876,7 @@ __device__ __forceinline__ int dep_rank(const AsmArgs& A, int k, const Geo& G, i
 // identity (Petsc_Functions.cpp:42051-42056) and still feeds its
 // neighbours; no row of a zero-porosity block is written here otherwise.
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, const int64_t* browptr,
+__global__ __launch_bounds__(64) void k_red_own(AsmArgs A, const double* dred, const int64_t* browptr,
                                                  int32_t* bcols, double* bvals, double* rpart) {
     using D1 = DualV<1>;
     constexpr int NP = NC + 1;
@@ -2196,14 +2196,16 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
         OSR_HIP(hipMalloc((void**)&a->dred, (size_t)nb * osr::JacShape<NC>::RF * (NP + 1) * sizeof(double)));
     if (!a->rpart) OSR_HIP(hipMalloc((void**)&a->rpart, (size_t)nb * 8 * NP * sizeof(double)));
     const unsigned g1 = (unsigned)((nb + 255) / 256);
-    const unsigned gc = (unsigned)((nb * MAXC + 255) / 256);
     const unsigned gq = (unsigned)((nb * (NP + 1) + 255) / 256);
-    hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3(gc), dim3(256), 0, st, A, a->wloc);
+    // the one-wave-per-SIMD kernels run in one-wave workgroups, so each SIMD
+    // takes its next wave as soon as its own finishes
+    hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3((unsigned)((nb * MAXC + 63) / 64)), dim3(64), 0, st, A, a->wloc);
     hipLaunchKernelGGL((osr::k_red_local<NC, false>), dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256),
                        0, st, A, (const double*)a->wloc, F, (const int64_t*)nullptr, (const double*)nullptr, S, t);
     hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)S, (const double*)t,
                        a->dred);
-    hipLaunchKernelGGL(osr::k_red_own<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)a->dred, browptr, bcols,
+    hipLaunchKernelGGL(osr::k_red_own<NC>, dim3((unsigned)((nb * (NP + 1) + 63) / 64)), dim3(64), 0, st, A,
+                       (const double*)a->dred, browptr, bcols,
                        bvals, a->rpart);
     hipLaunchKernelGGL(osr::k_red_rhs<NC>, dim3(g1), dim3(256), 0, st, A, F, (const double*)a->rpart, brhs);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
