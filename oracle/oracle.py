@@ -250,6 +250,26 @@ def jacobian(fluid, grid, state):
     return rowptr, cols[:nnz].copy(), vals[:nnz].copy()
 
 
+def jacobian_ref(fluid, grid, state):
+    """The reference's hand-written Jacobian (IteratorJacobian restated) as
+    (rowptr, cols, vals) in the reference layout."""
+    a = _Asm(fluid, grid, state)
+    _, off = layout(fluid, grid, state)
+    nrow = int(off[-1])
+    rowptr = np.zeros(nrow + 1, np.int64)
+    cap = nrow * (8 * (2 * a.nc + 4) + 1)
+    cols = np.zeros(cap, np.int32)
+    vals = np.zeros(cap)
+    L = lib()
+    L.or_jacobian_ref.restype = C.c_int64
+    nnz = L.or_jacobian_ref(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state),
+                            rowptr.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+                            _p(vals), C.c_int64(cap))
+    assert nnz != -1, "Jacobian capacity too small"
+    assert nnz != -2, "a row emits a column outside the matrix"
+    return rowptr, cols[:nnz].copy(), vals[:nnz].copy()
+
+
 def pack(fluid, grid, state):
     a = _Asm(fluid, grid, state)
     _, off = layout(fluid, grid, state)
