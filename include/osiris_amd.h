@@ -83,6 +83,27 @@ int osr_flash_device(osr_ctx *ctx, int64_t n, const double *T, double T0,
                      const double *P, const double *z, double sw,
                      const osr_flash_out *out, int mode, void *stream);
 
+/* Allocation-free variant of osr_flash_device, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as osr_flash_device; the parked-cell store
+ * of the persistent and mixed forms lives in a fixed 64 MiB workspace per
+ * device (allocated when the library is loaded, kept until the process ends)
+ * instead of a buffer grown on demand.  Calls no hipMalloc/hipFree.  Returns
+ * OSR_E_SIZE, launching nothing, when n exceeds
+ * osr_flash_noalloc_max_cells(nc, mode).  Every call on one device shares
+ * that workspace and the context's queue counters: calls that overlap on one
+ * device (several streams, or several contexts on the same device) race. */
+#define OSR_E_SIZE (-4)     /* input larger than a fixed-size workspace allows */
+int osr_flash_device_noalloc(osr_ctx *ctx, int64_t n, const double *T, double T0,
+                             const double *P, const double *z, double sw,
+                             const osr_flash_out *out, int mode, void *stream);
+
+/* Largest n osr_flash_device_noalloc accepts for nc components and `mode`:
+ * 8 Mi doubles / (nc + 5) persistent, / (2 nc + 9) mixed, unbounded (INT64_MAX)
+ * plain.  Host-only; OSR_E_NC / OSR_E_ARG for an unsupported nc / mode. */
+int64_t osr_flash_noalloc_max_cells(int32_t nc, int32_t mode);
+
 /* Saturation pressure of n cells: kind[c] = 1 runs Bubl_Succ
  * (Minor_NR.cpp:491-871) on liquid x with stored vapour y, kind[c] = -1 runs
  * Dew_Succ (Minor_NR.cpp:873-1249) on vapour y, 0 skips (psat = -1).  P is

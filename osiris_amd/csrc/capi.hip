@@ -75,6 +75,10 @@ int osr_ctx_create(const osr_fluid* fluid, osr_ctx** out) {
         osr_ctx_destroy(c);
         return OSR_E_HIP;
     }
+    // The allocation-free flash's workspace address on this device; if it
+    // cannot be had, d_scratch stays null and osr_flash_device_noalloc
+    // returns OSR_E_HIP (the allocating entry points are unaffected).
+    if (osr::flash_scratch_init(c->device, &c->d_scratch) != OSR_OK) c->d_scratch = nullptr;
     *out = c;
     return OSR_OK;
 }
@@ -97,7 +101,19 @@ int osr_ctx_destroy(osr_ctx* ctx) {
 int osr_flash_device(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
                      const double* z, double sw, const osr_flash_out* out, int mode,
                      void* stream) {
-    return osr::launch_flash(ctx, n, T, T0, P, z, sw, out, mode, (hipStream_t)stream);
+    return osr::launch_flash(ctx, n, T, T0, P, z, sw, out, mode, false, (hipStream_t)stream);
+}
+
+int osr_flash_device_noalloc(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
+                             const double* z, double sw, const osr_flash_out* out, int mode,
+                             void* stream) {
+    return osr::launch_flash(ctx, n, T, T0, P, z, sw, out, mode, true, (hipStream_t)stream);
+}
+
+int64_t osr_flash_noalloc_max_cells(int32_t nc, int32_t mode) {
+    if (!osr::nc_supported(nc)) return OSR_E_NC;
+    if (mode != OSR_FLASH_PLAIN && mode != OSR_FLASH_PERSISTENT && mode != OSR_FLASH_MIXED) return OSR_E_ARG;
+    return osr::flash_noalloc_max_cells(nc, mode);
 }
 
 int osr_psat_device(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,

@@ -787,8 +787,42 @@ __global__ void k_zero_u64(unsigned long long* p, int count) {
 
 // Workspace for parked cells: park store [NC + 5][n], then (mixed) kb
 // [NC][n], itb [2][n], list2 [n], list3 [n] -- every cell parks at most once.
+static size_t park_words(int nc, int64_t n, bool mixed) {
+    return (size_t)(nc + 5) * (size_t)n + (mixed ? (size_t)(nc + 4) * (size_t)n : 0);
+}
+
+// Fixed park workspace of the allocation-free entry point
+// (osr_flash_device_noalloc).  One copy per device, held until the process
+// ends.  Its device address is taken once per device, with that device
+// current, when a context is created (flash_scratch_init), and only that
+// address is ever handed to kernels or copies: on the host, `scratch` names a
+// host-side shadow, not device memory.
+static __device__ __align__(256) double scratch[FLASH_SCRATCH_WORDS];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static double* g_scratch_dev[SCRATCH_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+int flash_scratch_init(int device, double** out) {
+    if (!out || device < 0 || device >= SCRATCH_MAX_DEVICES) return OSR_E_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return OSR_E_HIP;
+        g_scratch_dev[device] = (double*)p;
+    }
+    *out = g_scratch_dev[device];
+    return OSR_OK;
+}
+
+int64_t flash_noalloc_max_cells(int nc, int mode) {
+    if (mode == OSR_FLASH_PLAIN) return INT64_MAX;
+    const size_t per_cell = park_words(nc, 1, mode == OSR_FLASH_MIXED);
+    return (int64_t)(FLASH_SCRATCH_WORDS / per_cell);
+}
+
 static int ensure_park(osr_ctx* ctx, int nc, int64_t n, bool mixed) {
-    const size_t words = (size_t)(nc + 5) * (size_t)n + (mixed ? (size_t)(nc + 4) * (size_t)n : 0);
+    const size_t words = park_words(nc, n, mixed);
     const size_t need = words * sizeof(double);
     if (need > ctx->park_bytes) {
         if (ctx->d_park) (void)hipFree(ctx->d_park);
@@ -801,7 +835,7 @@ static int ensure_park(osr_ctx* ctx, int nc, int64_t n, bool mixed) {
 }
 
 template <int NC>
-static int launch_nc(osr_ctx* ctx, const FlashArgs& a, int mode, hipStream_t stream) {
+static int launch_nc(osr_ctx* ctx, const FlashArgs& a, int mode, bool noalloc, hipStream_t stream) {
     const int block = 256;
     if (mode == OSR_FLASH_PLAIN) {
         const int64_t grid = (a.n + block - 1) / block;
@@ -821,8 +855,15 @@ static int launch_nc(osr_ctx* ctx, const FlashArgs& a, int mode, hipStream_t str
     FlashArgs b = a;
     const int park_after = mixed ? (ctx->mixed_warm_iters < 1 ? 1 : ctx->mixed_warm_iters) : ctx->park_after;
     if (park_after > 0) {
-        OSR_HIP_RET(ensure_park(ctx, NC, a.n, mixed));
-        double* base = (double*)ctx->d_park;
+        double* base;
+        if (noalloc) {
+            // size checked against FLASH_SCRATCH_WORDS by launch_flash
+            if (!ctx->d_scratch) return OSR_E_HIP;
+            base = ctx->d_scratch;
+        } else {
+            OSR_HIP_RET(ensure_park(ctx, NC, a.n, mixed));
+            base = (double*)ctx->d_park;
+        }
         const int64_t n = a.n;
         b.park_after = park_after;
         b.park = base;
@@ -876,10 +917,11 @@ bool nc_supported(int nc) {
 
 int launch_flash(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
                  const double* z, double sw, const osr_flash_out* out, int mode,
-                 hipStream_t stream) {
+                 bool noalloc, hipStream_t stream) {
     if (!ctx || n < 0 || !out) return OSR_E_ARG;
     if (mode != OSR_FLASH_PLAIN && mode != OSR_FLASH_PERSISTENT && mode != OSR_FLASH_MIXED)
         return OSR_E_ARG;
+    if (noalloc && n > flash_noalloc_max_cells(ctx->nc, mode)) return OSR_E_SIZE;
     if (n == 0) return OSR_OK;
     if (!P || !z || (!T && !(T0 > 0))) return OSR_E_ARG;
     FlashArgs a;
@@ -904,14 +946,14 @@ int launch_flash(osr_ctx* ctx, int64_t n, const double* T, double T0, const doub
     a.tol32 = ctx->mixed_tol32;
     a.max32 = ctx->mixed_max32;
     switch (ctx->nc) {
-        case 2: return launch_nc<2>(ctx, a, mode, stream);
-        case 3: return launch_nc<3>(ctx, a, mode, stream);
-        case 4: return launch_nc<4>(ctx, a, mode, stream);
-        case 5: return launch_nc<5>(ctx, a, mode, stream);
-        case 6: return launch_nc<6>(ctx, a, mode, stream);
-        case 8: return launch_nc<8>(ctx, a, mode, stream);
-        case 10: return launch_nc<10>(ctx, a, mode, stream);
-        case 12: return launch_nc<12>(ctx, a, mode, stream);
+        case 2: return launch_nc<2>(ctx, a, mode, noalloc, stream);
+        case 3: return launch_nc<3>(ctx, a, mode, noalloc, stream);
+        case 4: return launch_nc<4>(ctx, a, mode, noalloc, stream);
+        case 5: return launch_nc<5>(ctx, a, mode, noalloc, stream);
+        case 6: return launch_nc<6>(ctx, a, mode, noalloc, stream);
+        case 8: return launch_nc<8>(ctx, a, mode, noalloc, stream);
+        case 10: return launch_nc<10>(ctx, a, mode, noalloc, stream);
+        case 12: return launch_nc<12>(ctx, a, mode, noalloc, stream);
         default: return OSR_E_NC;
     }
 }

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/osiris_amd.h"
 #include "eos_dev.hpp"
 
@@ -30,6 +32,9 @@ struct osr_ctx {
     // Saturation-pressure work list: [count (8 bytes) | cell indices int32[n]]
     void* d_plist = nullptr;
     size_t plist_bytes = 0;
+    // Device address of this device's fixed park workspace (flash.hip
+    // `scratch`), taken at context creation; used by osr_flash_device_noalloc.
+    double* d_scratch = nullptr;
     int mixed_warm_iters = 3;
     float mixed_tol32 = 1e-4f;
     int mixed_max32 = 150;
@@ -48,9 +53,13 @@ struct osr_ctx {
     } while (0)
 
 namespace osr {
+// Size of flash.hip's fixed park workspace, in doubles (64 MiB).
+constexpr size_t FLASH_SCRATCH_WORDS = (size_t)8 << 20;
 int launch_flash(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
                  const double* z, double sw, const osr_flash_out* out, int mode,
-                 hipStream_t stream);
+                 bool noalloc, hipStream_t stream);
+int flash_scratch_init(int device, double** out);
+int64_t flash_noalloc_max_cells(int nc, int mode);
 int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const double* P,
                 const int8_t* kind, const double* x, const double* y, double* psat,
                 double* xm, hipStream_t stream);

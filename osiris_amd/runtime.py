@@ -52,6 +52,9 @@ def lib():
         L.osr_flash_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_double, C.c_void_p,
                                        C.c_void_p, C.c_double, C.POINTER(_FlashOut), C.c_int,
                                        C.c_void_p]
+        L.osr_flash_device_noalloc.argtypes = L.osr_flash_device.argtypes
+        L.osr_flash_noalloc_max_cells.argtypes = [C.c_int32, C.c_int32]
+        L.osr_flash_noalloc_max_cells.restype = C.c_int64
         L.osr_psat_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_double, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p]
@@ -141,8 +144,11 @@ def alloc_flash_out(nc, n, device="cuda", full=True):
     return out
 
 
-def flash(ctx, T, P, z, sw=0.0, mode="persistent", out=None, stream=None):
-    """Flash n cells (SuccFalsh).  T: scalar or [n]; P: [n]; z: [nc, n]."""
+def flash(ctx, T, P, z, sw=0.0, mode="persistent", out=None, stream=None, noalloc=False):
+    """Flash n cells (SuccFalsh).  T: scalar or [n]; P: [n]; z: [nc, n].
+    noalloc=True calls osr_flash_device_noalloc (fixed per-device workspace,
+    no device allocation inside the library call; n is bounded by
+    osr_flash_noalloc_max_cells)."""
     import torch
     z = _as_dev(z, torch.float64)
     nc, n = z.shape
@@ -161,10 +167,13 @@ def flash(ctx, T, P, z, sw=0.0, mode="persistent", out=None, stream=None):
         out = alloc_flash_out(nc, n)
     fo = _FlashOut(*[(_ptr(out.get(k)).value if out.get(k) is not None else None)
                      for k, _ in _FlashOut._fields_])
-    rc = lib().osr_flash_device(ctx.h, n, _ptr(Tt), T0, _ptr(P), _ptr(z), float(sw),
-                                C.byref(fo), _MODES[mode] if isinstance(mode, str) else int(mode),
-                                _stream_ptr(stream))
-    _check(rc, "osr_flash_device")
+    fn = lib().osr_flash_device_noalloc if noalloc else lib().osr_flash_device
+    m = _MODES[mode] if isinstance(mode, str) else int(mode)
+    rc = fn(ctx.h, n, _ptr(Tt), T0, _ptr(P), _ptr(z), float(sw), C.byref(fo), m, _stream_ptr(stream))
+    if rc == -4:
+        raise OsrError(f"osr_flash_device_noalloc: {n} cells exceed the fixed workspace "
+                       f"({lib().osr_flash_noalloc_max_cells(nc, m)} cells at nc={nc})")
+    _check(rc, "osr_flash_device_noalloc" if noalloc else "osr_flash_device")
     return out
 
 

@@ -48,3 +48,16 @@ def test_ctx_create_rejects_bad_arguments():
     assert lib.osr_ctx_create(C.byref(rt._Fluid(7, 0, *ptrs)), C.byref(h)) == -3
     assert lib.osr_ctx_create(C.byref(rt._Fluid(7, 5, *ptrs)), C.byref(h)) == -1
     assert lib.osr_flash_device(None, 1, None, 300.0, None, None, 0.0, None, 1, None) == -1
+
+
+def test_noalloc_flash_limits():
+    # 8 Mi doubles of park workspace: (nc + 5) words per cell persistent,
+    # (2 nc + 9) mixed, no limit for the plain form.
+    lib = rt.lib()
+    for nc in (2, 3, 4, 5, 6, 8, 10, 12):
+        assert lib.osr_flash_noalloc_max_cells(nc, 1) == (8 << 20) // (nc + 5)
+        assert lib.osr_flash_noalloc_max_cells(nc, 2) == (8 << 20) // (2 * nc + 9)
+        assert lib.osr_flash_noalloc_max_cells(nc, 0) == 2 ** 63 - 1
+    assert lib.osr_flash_noalloc_max_cells(7, 1) == -3
+    assert lib.osr_flash_noalloc_max_cells(6, 3) == -1
+    assert lib.osr_flash_device_noalloc(None, 1, None, 300.0, None, None, 0.0, None, 1, None) == -1
