@@ -28,6 +28,14 @@
 // dE and DiffVisco are the exact partial derivatives of Calc_Visco's density
 // and LBC viscosity (implicit dZ), so they are taken here by forward-mode
 // differentiation of the same property functions the residual uses.
+//
+// Pins (tests/test_oracle_refjac.py): the column pattern against the row
+// code's structure and PJac's count; with no capillary pressure, diffusion or
+// gravity, equality with the exact Jacobian up to the reference's sign rules
+// and listed copy-paste rules; the transfer terms' sign rule; fugacity and
+// constraint rows exact.  Parity unpinned (the restatement alone stands for
+// them): the Fickian and diffusive-transfer derivative terms, the z-face
+// gravity terms and the terms that depend on capillary pressure or IFT.
 #include "osr_model.hpp"
 
 namespace {
