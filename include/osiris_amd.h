@@ -250,6 +250,25 @@ int osr_jacobian_rowptr_device(osr_asm *a, const osr_state_dev *s, int64_t *rowp
 int osr_jacobian_device(osr_asm *a, const osr_state_dev *s, const int64_t *rowptr, int32_t *cols,
                         double *vals, void *stream);
 
+/* The reference's own Jacobian: IteratorJacobian (Petsc_Functions.cpp:
+ * 6898-42073) as the FormJacobian row loop (PetscAUX.cpp:106-173) assembles
+ * it with MatSetValues(ADD_VALUES) into PJac's matrix (Init_Calcs.cpp:
+ * 1013-1116): the columns each row's code emits (own block, six neighbours
+ * and the other continuum's block whole for the balance rows, less a gas
+ * block's own Sw; {P, Sw} of the same blocks for the water row; x, y, P of
+ * the own block for fugacity rows; the constraint rows' columns with the
+ * diagonal placeholder of :7004-7006), ascending, and the values of its
+ * hand rules: dRelPerm with the sign clamps and frozen IFT (Do_Cycle.cpp:
+ * 163-256), no capillary-pressure derivatives, dE-based dZ, DiffVisco and
+ * 1e4-scaled DifDiffusion (Mass_Tran.cpp:107-399), its sign conventions for
+ * neighbour saturation and transfer columns and its per-face copy-paste
+ * assignments.  Same state and layout contract as osr_jacobian_device.
+ * osr_jacobian_ref_rowptr_device: rowptr [nrows+1] (device), *nnz (host). */
+int osr_jacobian_ref_rowptr_device(osr_asm *a, const osr_state_dev *s, int64_t *rowptr, int64_t *nnz,
+                                   void *stream);
+int osr_jacobian_ref_device(osr_asm *a, const osr_state_dev *s, const int64_t *rowptr, int32_t *cols,
+                            double *vals, void *stream);
+
 /* UpdateAll (PetscAUX.cpp:176-498): unknown vector X [nrows] (device) ->
  * state, in place: mole fractions clamped to [0, 1] and renormalised, the
  * pressure limiter (+-0.01 Pa when the new pressure exceeds the old one by

@@ -2020,6 +2020,8 @@ __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
 }  // namespace osr
 
+#include "refjac_dev.hpp"
+
 // ============================================================== host side
 struct osr_asm {
     osr_ctx* ctx = nullptr;
@@ -2146,6 +2148,35 @@ int jacobian_nc(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, int32
     const int64_t gn = (nb * 7 * MAXC + 255) / 256;
     hipLaunchKernelGGL(osr::k_jac_nb<NC>, dim3((unsigned)gn), dim3(256), 0, st, A, (const double*)a->dprops,
                        rowptr, cols, vals);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+// The reference's hand-written Jacobian (IteratorJacobian): its pattern
+// depends on the labels only; its values on the property sweep, the
+// reference-rule derivative table (in the Jacobian workspace) and the faces.
+template <int NC>
+int rj_rowptr_nc(osr_asm* a, const osr_state_dev* s, int64_t* rowptr, int64_t nrows, hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    const int nb = 2 * a->n;
+    hipLaunchKernelGGL(osr::k_rj_rownnz<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, A, rowptr);
+    hipLaunchKernelGGL(osr::k_set_i64, dim3(1), dim3(1), 0, st, rowptr + nrows, (int64_t)0);
+    return exclusive_scan(a, rowptr, rowptr, nrows + 1, st);
+}
+
+template <int NC>
+int rj_jacobian_nc(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, int32_t* cols, double* vals,
+                   hipStream_t st) {
+    osr::AsmArgs A = make_args(a, s);
+    int rc = run_props<NC>(a, A, st);
+    if (rc) return rc;
+    constexpr int NF = osr::JacShape<NC>::NF, MAXC = osr::JacShape<NC>::MAXC;
+    static_assert(osr::RJ<NC>::NF <= NF * MAXC, "reference-rule table exceeds the Jacobian workspace");
+    const int64_t nb = 2 * (int64_t)a->n;
+    if (!a->dprops) OSR_HIP(hipMalloc((void**)&a->dprops, (size_t)nb * NF * MAXC * sizeof(double)));
+    hipLaunchKernelGGL(osr::k_rj_cells<NC>, dim3((unsigned)((nb * (NC + 1) + 255) / 256)), dim3(256), 0, st, A,
+                       a->dprops);
+    hipLaunchKernelGGL(osr::k_rj_rows<NC>, dim3((unsigned)((nb * MAXC + 127) / 128)), dim3(128), 0, st, A,
+                       (const double*)a->dprops, rowptr, cols, vals);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -2494,6 +2525,41 @@ int osr_jacobian_device(osr_asm* a, const osr_state_dev* s, const int64_t* rowpt
     if (rc) return rc;
     if (!rowptr || !cols || !vals) return OSR_E_ARG;
 #define CALL(NC) jacobian_nc<NC>(a, s, rowptr, cols, vals, (hipStream_t)stream)
+    OSR_NC_SWITCH(a->nc, CALL)
+#undef CALL
+}
+
+int osr_jacobian_ref_rowptr_device(osr_asm* a, const osr_state_dev* s, int64_t* rowptr, int64_t* nnz,
+                                   void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!rowptr) return OSR_E_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    switch (a->nc) {
+        case 2: rc = rj_rowptr_nc<2>(a, s, rowptr, a->nrows, st); break;
+        case 3: rc = rj_rowptr_nc<3>(a, s, rowptr, a->nrows, st); break;
+        case 4: rc = rj_rowptr_nc<4>(a, s, rowptr, a->nrows, st); break;
+        case 5: rc = rj_rowptr_nc<5>(a, s, rowptr, a->nrows, st); break;
+        case 6: rc = rj_rowptr_nc<6>(a, s, rowptr, a->nrows, st); break;
+        case 8: rc = rj_rowptr_nc<8>(a, s, rowptr, a->nrows, st); break;
+        case 10: rc = rj_rowptr_nc<10>(a, s, rowptr, a->nrows, st); break;
+        case 12: rc = rj_rowptr_nc<12>(a, s, rowptr, a->nrows, st); break;
+        default: return OSR_E_NC;
+    }
+    if (rc) return rc;
+    if (nnz) {
+        OSR_HIP(hipMemcpyAsync(nnz, rowptr + a->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        OSR_HIP(hipStreamSynchronize(st));
+    }
+    return OSR_OK;
+}
+
+int osr_jacobian_ref_device(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, int32_t* cols, double* vals,
+                            void* stream) {
+    int rc = check_state(a, s);
+    if (rc) return rc;
+    if (!rowptr || !cols || !vals) return OSR_E_ARG;
+#define CALL(NC) rj_jacobian_nc<NC>(a, s, rowptr, cols, vals, (hipStream_t)stream)
     OSR_NC_SWITCH(a->nc, CALL)
 #undef CALL
 }

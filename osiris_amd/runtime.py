@@ -302,6 +302,8 @@ def _bind_asm(L):
                                              C.POINTER(C.c_int64), C.c_void_p]
     L.osr_jacobian_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]
+    L.osr_jacobian_ref_rowptr_device.argtypes = L.osr_jacobian_rowptr_device.argtypes
+    L.osr_jacobian_ref_device.argtypes = L.osr_jacobian_device.argtypes
     L.osr_props_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
     # osr_state_mut_dev has the field layout of osr_state_dev
     L.osr_unpack_device.argtypes = [C.c_void_p, C.POINTER(_StateDev), C.c_void_p, C.c_void_p]
@@ -415,22 +417,26 @@ class Assembler:
                                          _stream_ptr(stream)), "osr_residual_device")
         return F
 
-    def jacobian(self, d, stream=None):
-        """Returns (rowptr, cols, vals) as device tensors."""
+    def jacobian(self, d, stream=None, reference=False):
+        """Returns (rowptr, cols, vals) as device tensors: the exact
+        derivative of the residual (osr_jacobian_device) or, with
+        reference=True, the reference's hand-written Jacobian in its own
+        pattern (osr_jacobian_ref_device)."""
         import torch
         if self.nrows is None:
             self.layout(d, stream)
+        L = lib()
+        rp_fn, fn = ((L.osr_jacobian_ref_rowptr_device, L.osr_jacobian_ref_device) if reference else
+                     (L.osr_jacobian_rowptr_device, L.osr_jacobian_device))
         rowptr = torch.empty(self.nrows + 1, dtype=torch.int64, device="cuda")
         nnz = C.c_int64()
         sd = self._sd(d)
-        _check(lib().osr_jacobian_rowptr_device(self.h, C.byref(sd), C.c_void_p(rowptr.data_ptr()),
-                                                C.byref(nnz), _stream_ptr(stream)),
-               "osr_jacobian_rowptr_device")
+        _check(rp_fn(self.h, C.byref(sd), C.c_void_p(rowptr.data_ptr()), C.byref(nnz), _stream_ptr(stream)),
+               rp_fn.__name__)
         cols = torch.empty(nnz.value, dtype=torch.int32, device="cuda")
         vals = torch.empty(nnz.value, dtype=torch.float64, device="cuda")
-        _check(lib().osr_jacobian_device(self.h, C.byref(sd), C.c_void_p(rowptr.data_ptr()),
-                                         C.c_void_p(cols.data_ptr()), C.c_void_p(vals.data_ptr()),
-                                         _stream_ptr(stream)), "osr_jacobian_device")
+        _check(fn(self.h, C.byref(sd), C.c_void_p(rowptr.data_ptr()), C.c_void_p(cols.data_ptr()),
+                  C.c_void_p(vals.data_ptr()), _stream_ptr(stream)), fn.__name__)
         return rowptr, cols, vals
 
     def pack(self, d, stream=None):
