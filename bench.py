@@ -720,12 +720,24 @@ def srk_oracle_parity(O, fl, cells, res, threads):
     return out
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_newton_sample(rt, synth, torch):
-    """CPU baseline of the Newton-iteration step: the oracle's flash of every
-    cell, PhaseCtrl of both continua, residual and reference-layout Jacobian
-    on a 16x16x8 sample grid of the same workload (10 components), one
-    thread (the oracle's assembly is a scalar restatement); the GPU's CSR
-    Jacobian and residual of the same sample checked against it."""
+    """CPU baseline of the Newton-iteration step on the host cores: the
+    oracle's flash of every cell, PhaseCtrl of both continua, residual and
+    the reference's own Jacobian (IteratorJacobian restated, assembled row by
+    row as FormJacobian does, PetscAUX.cpp:84-91, 135-143), on a 32x32x16
+    dual-porosity grid of the same workload (10 components), once on one
+    thread and once on every host core.  The GPU's CSR Jacobian and residual
+    are then checked against the oracle on a 16x16x8 sample."""
     from oracle import oracle as O
     from osiris_amd.batch import NewtonBatch
     fl = synth.fluid_pseudo(NEWTON_NC, synth.PR, NEWTON_FLUID_SEED)
@@ -737,27 +749,51 @@ def cpu_newton_sample(rt, synth, torch):
         r["so"], r["sg"] = nd * (1 - sw), (1 - nd) * (1 - sw)
         return r
 
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = min(ncores, 16)
+    gridB, stB = synth.dual_porosity_case(fl, 32, 32, 16, 98, oflash, p_range=(8e6, 20e6))
+
+    def step(grid, st, threads):
+        n = grid.n
+        T = np.full(2 * n, grid.T)
+        P = np.concatenate([st.Po_m, st.Po_f])
+        z = np.concatenate([st.z_m, st.z_f], axis=1)
+        tp = [time.perf_counter()]
+        O.flash_batch(fl, T, P, z, nthreads=threads)
+        tp.append(time.perf_counter())
+        pr = O.props(fl, grid, st)
+        tp.append(time.perf_counter())
+        for k, (lab, Po, x, y, S) in enumerate(((st.label_m, st.Po_m, st.x_m, st.y_m, st.S_m),
+                                                (st.label_f, st.Po_f, st.x_f, st.y_f, st.S_f))):
+            O.phase_ctrl(fl, np.full(n, grid.T), lab.copy(), Po, pr[k, :, 3].copy(), x.copy(), y.copy(),
+                         S.copy(), nthreads=threads)
+        tp.append(time.perf_counter())
+        O.residual(fl, grid, st)
+        tp.append(time.perf_counter())
+        O.jacobian_ref(fl, grid, st, nthreads=threads)
+        tp.append(time.perf_counter())
+        return tp[-1] - tp[0], dict(zip(("flash", "property_sweep", "phase_test", "residual",
+                                         "reference_jacobian"), np.diff(tp).tolist()))
+
+    one_s, one_ph = step(gridB, stB, 1)
+    all_s, all_ph = step(gridB, stB, ncores)
+    nB = 2 * gridB.n
+    base = {"value": nB / all_s, "unit": "cells/s", "cores": ncores, "kind": "port",
+            "sample": f"32x32x16 dual-porosity grid of the same workload ({nB} cells, 10 components): oracle "
+                      "flash of every cell + PhaseCtrl + residual + the reference's Jacobian "
+                      "(IteratorJacobian restated, row by row), OpenMP over cells and blocks",
+            "seconds": all_s, "phase_seconds": all_ph,
+            "single_core": {"value": nB / one_s, "seconds": one_s, "phase_seconds": one_ph},
+            "cpu_model": _cpu_model(),
+            "note": "property sweep and residual run on one thread in both legs"}
+
     grid, st = synth.dual_porosity_case(fl, 16, 16, 8, 99, oflash, p_range=(8e6, 20e6))
     n = grid.n
-    T = np.full(2 * n, grid.T)
-    P = np.concatenate([st.Po_m, st.Po_f])
-    z = np.concatenate([st.z_m, st.z_f], axis=1)
-    tp = [time.perf_counter()]
-    O.flash_batch(fl, T, P, z, nthreads=1)
-    tp.append(time.perf_counter())
-    pr = O.props(fl, grid, st)
-    tp.append(time.perf_counter())
-    for k, (lab, Po, x, y, S) in enumerate(((st.label_m, st.Po_m, st.x_m, st.y_m, st.S_m),
-                                            (st.label_f, st.Po_f, st.x_f, st.y_f, st.S_f))):
-        O.phase_ctrl(fl, np.full(n, grid.T), lab.copy(), Po, pr[k, :, 3].copy(), x.copy(), y.copy(), S.copy(),
-                     nthreads=1)
-    tp.append(time.perf_counter())
     F_or = O.residual(fl, grid, st)
-    tp.append(time.perf_counter())
     rp, cols, vals = O.jacobian(fl, grid, st)
-    tp.append(time.perf_counter())
-    cpu_s = tp[-1] - tp[0]
-    phases = dict(zip(("flash", "property_sweep", "phase_test", "residual", "jacobian"), np.diff(tp).tolist()))
     # the GPU on the same sample (CSR form), against the oracle
     ctx = rt.Context(fl)
     b = NewtonBatch(ctx, fl, [(grid, st)], form="csr", flash_mode="plain")
@@ -781,14 +817,35 @@ def cpu_newton_sample(rt, synth, torch):
     j_entry = float((dj[big_entry] / np.abs(vals[big_entry])).max())
     j_floor = float((dj / np.maximum(np.abs(vals), 1e-8 * rowmax[rows])).max())
     b.close()
-    return {"value": 2 * n / cpu_s, "unit": "cells/s", "cores": 1, "kind": "port",
-            "sample": f"16x16x8 dual-porosity grid of the same workload ({2 * n} cells, 10 components): oracle "
-                      "flash of every cell + PhaseCtrl + residual + reference-layout Jacobian, one thread",
-            "seconds": cpu_s, "phase_seconds": phases}, {"jacobian_pattern_identical": same_pattern,
+    # the analytic path of the headline (reduced blocks straight from the
+    # state) on the same sample, against the Schur complement of the
+    # oracle's CSR Jacobian and residual
+    asm = rt.Assembler(ctx, fl, grid)
+    d = asm.device_state(st)
+    asm.layout(d)
+    Fd = asm.residual(d)
+    red = asm.reduced_direct(d, Fd)
+    torch.cuda.synchronize()
+    blocks, rhs, _, _, _, _ = O.reduced_from_csr(fl, grid, st, rp, cols, vals, F_or)
+    browptr, bcols, bvals, brhs = (x.cpu().numpy() for x in red[:4])
+    rerr = []
+    for a in range(2 * n):
+        deps = bcols[browptr[a]:browptr[a + 1]].tolist()
+        ref = np.stack([blocks[(a, dd)] for dd in deps])             # [m, np1, np1]
+        scale = np.maximum(np.abs(ref).max(axis=(0, 2)), np.abs(rhs[a]))
+        e = np.abs(bvals[browptr[a]:browptr[a + 1]] - ref).max(axis=(0, 2))
+        rerr.append(np.maximum(e, np.abs(brhs[a] - rhs[a])) / np.maximum(scale, 1e-300))
+    rerr = np.concatenate(rerr)
+    asm.close()
+    return base, {"jacobian_pattern_identical": same_pattern,
                                 "residual_max_row_scaled_err": f_err,
                                 "jacobian_max_rowwise_rel_err": j_row,
                                 "jacobian_max_rel_err_entries_above_1e-4_of_row": j_entry,
-                                "jacobian_max_rel_err_floored_1e-8_of_row": j_floor}
+                                "jacobian_max_rel_err_floored_1e-8_of_row": j_floor,
+                                "reduced_direct_vs_oracle_schur": {
+                                    "max_row_scaled_err": float(rerr.max()),
+                                    "p99": float(np.quantile(rerr, 0.99)),
+                                    "rows_above_1e-8": int((rerr > 1e-8).sum()), "rows": int(rerr.size)}}
 
 
 def main():

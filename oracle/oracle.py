@@ -250,9 +250,10 @@ def jacobian(fluid, grid, state):
     return rowptr, cols[:nnz].copy(), vals[:nnz].copy()
 
 
-def jacobian_ref(fluid, grid, state):
+def jacobian_ref(fluid, grid, state, nthreads=1):
     """The reference's hand-written Jacobian (IteratorJacobian restated) as
-    (rowptr, cols, vals) in the reference layout."""
+    (rowptr, cols, vals) in the reference layout; nthreads OpenMP threads
+    (<= 0: all), same output for any count."""
     a = _Asm(fluid, grid, state)
     _, off = layout(fluid, grid, state)
     nrow = int(off[-1])
@@ -261,13 +262,79 @@ def jacobian_ref(fluid, grid, state):
     cols = np.zeros(cap, np.int32)
     vals = np.zeros(cap)
     L = lib()
-    L.or_jacobian_ref.restype = C.c_int64
-    nnz = L.or_jacobian_ref(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state),
-                            rowptr.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
-                            _p(vals), C.c_int64(cap))
+    L.or_jacobian_ref_mt.restype = C.c_int64
+    nnz = L.or_jacobian_ref_mt(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state),
+                               rowptr.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+                               _p(vals), C.c_int64(cap), C.c_int(nthreads))
     assert nnz != -1, "Jacobian capacity too small"
     assert nnz != -2, "a row emits a column outside the matrix"
     return rowptr, cols[:nnz].copy(), vals[:nnz].copy()
+
+
+def reduced_split(nc, label):
+    """Local indices of one block's (primary unknowns, secondary unknowns,
+    balance rows, local rows) in the reference layout: two-phase primaries
+    x_0..x_{nc-3}, P, Sw, So; single-phase x_0..x_{nc-2} (or y), P, Sw."""
+    if label == 0:
+        prim = list(range(nc - 2)) + [2 * nc, 2 * nc + 1, 2 * nc + 2]
+        sec = [nc - 2, nc - 1] + list(range(nc, 2 * nc)) + [2 * nc + 3]
+        mrow = list(range(nc, 2 * nc + 1))
+        lrow = list(range(nc)) + [2 * nc + 1, 2 * nc + 2, 2 * nc + 3]
+    else:
+        prim = list(range(nc - 1)) + [nc, nc + 1]
+        sec = [nc - 1, nc + 2]
+        mrow = list(range(nc + 1))
+        lrow = [nc + 1, nc + 2]
+    return prim, sec, mrow, lrow
+
+
+def reduced_from_csr(fluid, grid, state, rp, cols, vals, F):
+    """The reduced (nc+1)^2 Newton system as the Schur complement of a
+    reference-layout CSR Jacobian (dense per block): for every block,
+    S = J_ls^-1 J_lp and t = J_ls^-1 F_l of its local rows, and for every
+    pair (a, d) of blocks coupled through a balance row the block
+    J_mp,ad - J_ms,ad S_d; rhs_a = -F_m,a + sum_d J_ms,ad t_d.  Returns
+    (blocks {(a, d): [nc+1, nc+1]}, rhs [2n, nc+1], S {b: array}, t {b:
+    array}, prim {b: global column indices}, sec {b: ...})."""
+    import scipy.sparse as sp
+    nc, n = fluid.nc, grid.n
+    _, off = layout(fluid, grid, state)
+    N = int(off[-1])
+    J = sp.csr_matrix((vals, cols, rp), shape=(N, N))
+    labs = np.concatenate([state.label_m, state.label_f]).astype(int)
+    P, Sc, M, Lr = {}, {}, {}, {}
+    S, t = {}, {}
+    for b in range(2 * n):
+        pr, se, mr, lr = reduced_split(nc, labs[b])
+        o = int(off[b])
+        P[b], Sc[b] = [o + i for i in pr], [o + i for i in se]
+        M[b], Lr[b] = [o + i for i in mr], [o + i for i in lr]
+        Jl = J[Lr[b]][:, o:o + int(off[b + 1] - off[b])].toarray()
+        S[b] = np.linalg.solve(Jl[:, se], Jl[:, pr])
+        t[b] = np.linalg.solve(Jl[:, se], F[Lr[b]])
+    blocks = {}
+    rhs = np.zeros((2 * n, nc + 1))
+    blk_of = np.searchsorted(off, np.arange(N), side="right") - 1
+    for a in range(2 * n):
+        Jm = J[M[a]]
+        deps = sorted(set(blk_of[Jm.indices].tolist()))
+        cu = np.unique(Jm.indices)
+        Jd = Jm[:, cu].toarray()
+
+        def sub(idx):
+            out = np.zeros((Jd.shape[0], len(idx)))
+            pos = np.searchsorted(cu, idx)
+            ok = (pos < len(cu)) & (cu[np.minimum(pos, len(cu) - 1)] == idx)
+            out[:, ok] = Jd[:, pos[ok]]
+            return out
+
+        r = -F[M[a]]
+        for d in deps:
+            Js = sub(np.asarray(Sc[d]))
+            blocks[(a, d)] = sub(np.asarray(P[d])) - Js @ S[d]
+            r = r + Js @ t[d]
+        rhs[a] = r
+    return blocks, rhs, S, t, P, Sc
 
 
 def pack(fluid, grid, state):

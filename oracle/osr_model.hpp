@@ -769,7 +769,7 @@ struct Sweep {
 // the previous sweep; without them (rho_prev_m NULL) a first sweep with
 // zero densities supplies them (so its IFT is 0 and its gas phase pressure
 // equals the oil pressure), and the second sweep is the one used.
-void sweep(const Model& M, const or_state* s, Sweep& S) {
+void sweep(const Model& M, const or_state* s, Sweep& S, int nthreads = 1) {
     const int n = M.n;
     S.st = s;
     for (int k = 0; k < 2; k++) {
@@ -790,11 +790,13 @@ void sweep(const Model& M, const or_state* s, Sweep& S) {
             S.rhoprev[n + c] = tmp.ro[1];
         }
     }
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < 2; k++) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
         for (int c = 0; c < n; c++) {
             double rp[2];
             cell_props<double>(M, k, c, S.V[k][c], S.rp(n, c, rp), S.P[k][c]);
         }
+    }
 }
 
 }  // namespace

@@ -161,6 +161,10 @@ int64_t or_jacobian(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g,
  * the matrix. */
 int64_t or_jacobian_ref(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s,
                         int64_t *rowptr, int32_t *cols, double *vals, int64_t cap);
+/* The same with the cells, faces and blocks spread over nthreads OpenMP
+ * threads (<= 0: all); identical output. */
+int64_t or_jacobian_ref_mt(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s,
+                           int64_t *rowptr, int32_t *cols, double *vals, int64_t cap, int nthreads);
 int or_unpack(const or_grid *g, or_state_mut *s, int nc, const double *X);
 int or_pack(const or_grid *g, const or_state *s, int nc, double *X);
 

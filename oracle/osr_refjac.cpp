@@ -36,6 +36,8 @@
 // constraint rows exact.  Parity unpinned (the restatement alone stands for
 // them): the Fickian and diffusive-transfer derivative terms, the z-face
 // gravity terms and the terms that depend on capillary pressure or IFT.
+#include <omp.h>
+
 #include "osr_model.hpp"
 
 namespace {
@@ -102,7 +104,7 @@ void matrix_dkr(const Model& M, double sw, double sg, double bift, double* d) {
     if (d[2] > 0) d[2] = 0;
 }
 
-void ref_cells(const Model& M, const Sweep& Sw, int k, RefCells& R) {
+void ref_cells(const Model& M, const Sweep& Sw, int k, RefCells& R, int nthreads) {
     const int n = M.n, nc = M.nc;
     R.n = n;
     R.nc = nc;
@@ -113,6 +115,7 @@ void ref_cells(const Model& M, const Sweep& Sw, int k, RefCells& R) {
     R.dE.resize((size_t)n * (nc + 1) * 2); R.dMu.resize((size_t)n * (nc + 1) * 2);
     R.Dif.resize((size_t)n * nc * 2); R.dDif.resize((size_t)n * nc * (nc + 1) * 2);
     R.porpor.resize(n); R.dpor.resize(n); R.phi.resize(n);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int c = 0; c < n; c++) {
         const Vars<double>& v = Sw.V[k][c];
         const Props<double>& p = Sw.P[k][c];
@@ -1015,16 +1018,17 @@ extern "C" {
 // ADD_VALUES does on a zeroed matrix), sorted.  Returns the nnz, -1 if cap
 // is too small (rowptr still filled), -2 if a row emits a column outside
 // the matrix (the reference would fail there).
-int64_t or_jacobian_ref(const or_fluid* fl, const or_fluid_ext* fe, const or_grid* g, const or_state* st,
-                        int64_t* rowptr, int32_t* cols, double* vals, int64_t cap) {
+int64_t or_jacobian_ref_mt(const or_fluid* fl, const or_fluid_ext* fe, const or_grid* g, const or_state* st,
+                           int64_t* rowptr, int32_t* cols, double* vals, int64_t cap, int nthreads) {
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
     Model M;
     build_model(M, fl, fe, g);
     Sweep S;
-    sweep(M, st, S);
+    sweep(M, st, S, nthreads);
     const int n = M.n, nc = M.nc;
     RefCells R[2];
-    ref_cells(M, S, 0, R[0]);
-    ref_cells(M, S, 1, R[1]);
+    ref_cells(M, S, 0, R[0], nthreads);
+    ref_cells(M, S, 1, R[1], nthreads);
     Ctx X(M);
     X.R[0] = &R[0];
     X.R[1] = &R[1];
@@ -1044,6 +1048,7 @@ int64_t or_jacobian_ref(const or_fluid* fl, const or_fluid_ext* fe, const or_gri
     const int64_t nrows = X.boff[2 * (size_t)n];
     for (int k = 0; k < 2; k++) {
         X.F[k].resize(3 * (size_t)n);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
         for (int c = 0; c < n; c++)
             for (int d = 0; d < 3; d++) {
                 FaceGeo gg;
@@ -1051,51 +1056,75 @@ int64_t or_jacobian_ref(const or_fluid* fl, const or_fluid_ext* fe, const or_gri
                 X.F[k][(size_t)c * 3 + d] = ref_face(M, R[k], k, d, gg.a, gg.b, gg.GS, gg.GN, gg.hs, gg.hn);
             }
     }
+    // rows of each block in parallel (a block's rows into its own buffers),
+    // then concatenated in row order
+    const int64_t nb = 2 * (int64_t)n;
+    std::vector<std::vector<int32_t>> bcol(nb);
+    std::vector<std::vector<double>> bval(nb);
+    std::vector<std::vector<int32_t>> blen(nb);
+    bool bad = false;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16) reduction(|| : bad)
+    for (int64_t b = 0; b < nb; b++) {
+        const int k = (int)(b / n), c = (int)(b % n);
+        const int lab = R[k].lab[c];
+        const int bs = block_size(nc, lab);
+        RowOut row;
+        std::vector<int> ord;
+        for (int r = 0; r < bs; r++) {
+            const int64_t rowno = X.boff[(size_t)k * n + c] + r;
+            row.clear();
+            if (M.phi[k][c] < 1e-10) {
+                row.add(rowno, 1.0);   // zero-porosity block (:42051-42056)
+            } else if (lab == 0) {
+                if (r < nc) fugacity_row(X, k, c, r, row);
+                else if (r < 2 * nc - 1) balance_row(X, k, c, r - nc, false, row);
+                else if (r == 2 * nc - 1) balance_row(X, k, c, nc - 1, true, row);
+                else if (r == 2 * nc) water_row(X, k, c, row);
+                else constraint_row(X, k, c, r, row);
+            } else {
+                if (r < nc - 1) balance_row(X, k, c, r, false, row);
+                else if (r == nc - 1) balance_row(X, k, c, nc - 1, true, row);
+                else if (r == nc) water_row(X, k, c, row);
+                else constraint_row(X, k, c, r, row);
+            }
+            ord.resize(row.col.size());
+            for (size_t q = 0; q < ord.size(); q++) ord[q] = (int)q;
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b2) { return row.col[a] < row.col[b2]; });
+            int32_t len = 0;
+            for (size_t q = 0; q < ord.size();) {
+                const int64_t col = row.col[ord[q]];
+                if (col < 0 || col >= nrows) bad = true;
+                double v = 0.0;
+                size_t e = q;
+                while (e < ord.size() && row.col[ord[e]] == col) v += row.val[ord[e++]];
+                bcol[b].push_back((int32_t)col);
+                bval[b].push_back(v);
+                len++;
+                q = e;
+            }
+            blen[b].push_back(len);
+        }
+    }
     int64_t nnz = 0;
-    bool overflow = false, bad = false;
-    RowOut row;
-    std::vector<int> ord;
-    for (int k = 0; k < 2; k++)
-        for (int c = 0; c < n; c++) {
-            const int lab = R[k].lab[c];
-            const int bs = block_size(nc, lab);
-            for (int r = 0; r < bs; r++) {
-                const int64_t rowno = X.boff[(size_t)k * n + c] + r;
-                row.clear();
-                if (M.phi[k][c] < 1e-10) {
-                    row.add(rowno, 1.0);   // zero-porosity block (:42051-42056)
-                } else if (lab == 0) {
-                    if (r < nc) fugacity_row(X, k, c, r, row);
-                    else if (r < 2 * nc - 1) balance_row(X, k, c, r - nc, false, row);
-                    else if (r == 2 * nc - 1) balance_row(X, k, c, nc - 1, true, row);
-                    else if (r == 2 * nc) water_row(X, k, c, row);
-                    else constraint_row(X, k, c, r, row);
-                } else {
-                    if (r < nc - 1) balance_row(X, k, c, r, false, row);
-                    else if (r == nc - 1) balance_row(X, k, c, nc - 1, true, row);
-                    else if (r == nc) water_row(X, k, c, row);
-                    else constraint_row(X, k, c, r, row);
-                }
-                rowptr[rowno] = nnz;
-                ord.resize(row.col.size());
-                for (size_t q = 0; q < ord.size(); q++) ord[q] = (int)q;
-                std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return row.col[a] < row.col[b]; });
-                for (size_t q = 0; q < ord.size();) {
-                    const int64_t col = row.col[ord[q]];
-                    if (col < 0 || col >= nrows) bad = true;
-                    double v = 0.0;
-                    size_t e = q;
-                    while (e < ord.size() && row.col[ord[e]] == col) v += row.val[ord[e++]];
-                    if (nnz < cap) { cols[nnz] = (int32_t)col; vals[nnz] = v; }
-                    else overflow = true;
-                    nnz++;
-                    q = e;
-                }
+    bool overflow = false;
+    for (int64_t b = 0; b < nb; b++) {
+        int64_t rowno = X.boff[b], p = 0;
+        for (int32_t len : blen[b]) {
+            rowptr[rowno++] = nnz;
+            for (int32_t t = 0; t < len; t++, p++, nnz++) {
+                if (nnz < cap) { cols[nnz] = bcol[b][p]; vals[nnz] = bval[b][p]; }
+                else overflow = true;
             }
         }
+    }
     rowptr[nrows] = nnz;
     if (bad) return -2;
     return overflow ? -1 : nnz;
+}
+
+int64_t or_jacobian_ref(const or_fluid* fl, const or_fluid_ext* fe, const or_grid* g, const or_state* st,
+                        int64_t* rowptr, int32_t* cols, double* vals, int64_t cap) {
+    return or_jacobian_ref_mt(fl, fe, g, st, rowptr, cols, vals, cap, 1);
 }
 
 }  // extern "C"

@@ -208,3 +208,29 @@ def test_densities_match_independent_eos(oracle):
         Z = eos_np.pick(roots, False)
         assert pr[0, c, 5] == pytest.approx(Po / (Z * R * grid.T), rel=1e-9)
         del x
+
+
+def test_reduced_from_csr_solves_the_full_system(oracle):
+    """The oracle's Schur-complement reduction (the checker of the GPU's
+    reduced blocks): solving the reduced system and expanding with S, t
+    gives the full Newton update J du = -F."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from tests.asm_cases import c0_case, frozen
+    fl, grid, st = c0_case(oracle, nx=4, ny=3, nz=2, seed=1, sw_m=0.25, sw_f=0.05)
+    s = frozen(oracle, fl, grid, st)
+    rp, cols, vals = oracle.jacobian(fl, grid, s)
+    F = oracle.residual(fl, grid, s)
+    N = len(rp) - 1
+    blocks, rhs, S, t, P, Sc = oracle.reduced_from_csr(fl, grid, s, rp, cols, vals, F)
+    nb, np1 = 2 * grid.n, fl.nc + 1
+    R = np.zeros((nb * np1, nb * np1))
+    for (a, d), B in blocks.items():
+        R[a * np1:(a + 1) * np1, d * np1:(d + 1) * np1] = B
+    dp = np.linalg.solve(R, rhs.reshape(-1)).reshape(nb, np1)
+    du = np.zeros(N)
+    for b in range(nb):
+        du[P[b]] = dp[b]
+        du[Sc[b]] = -S[b] @ dp[b] - t[b]
+    ref = spla.spsolve(sp.csr_matrix((vals, cols, rp), shape=(N, N)).tocsc(), -F)
+    assert np.abs(du - ref).max() <= 1e-8 * np.abs(ref).max()
