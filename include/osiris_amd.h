@@ -52,6 +52,17 @@ typedef struct osr_ctx osr_ctx;
 int osr_ctx_create(const osr_fluid *fluid, osr_ctx **out);
 int osr_ctx_destroy(osr_ctx *ctx);
 
+/* Pin the context's workspaces (no counterpart in the reference): with
+ * frozen = 1, a call that would grow a workspace the context owns -- the
+ * flash's park store, the saturation-pressure work list, the phase-control
+ * queue -- returns OSR_E_SIZE (launching nothing) instead of freeing it and
+ * allocating a larger one, and osr_asm_set_workspace on an assembler of this
+ * context returns OSR_E_ARG; frozen = 0 lifts it.  For callers that capture
+ * calls of this context in a HIP graph: the graph keeps the workspaces'
+ * addresses, so they must not move while it can replay.  Eager calls made
+ * while such a graph replays still share its queue counters and race. */
+int osr_ctx_freeze_workspaces(osr_ctx *ctx, int frozen);
+
 /* Flash algorithm selector for osr_flash_device. */
 #define OSR_FLASH_PLAIN 0        /* one lane per cell, FP64, loop to convergence */
 #define OSR_FLASH_PERSISTENT 1   /* FP64 persistent kernel, lanes refilled from a work queue */

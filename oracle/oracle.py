@@ -250,6 +250,27 @@ def jacobian(fluid, grid, state):
     return rowptr, cols[:nnz].copy(), vals[:nnz].copy()
 
 
+def jacobian_blocks(fluid, grid, state, blocks):
+    """Rows of the listed blocks (k n + c) of the exact Jacobian, consecutively
+    in list order: (rowptr, cols, vals) with global column indices."""
+    a = _Asm(fluid, grid, state)
+    bs, off = layout(fluid, grid, state)
+    blocks = np.ascontiguousarray(blocks, dtype=np.int32)
+    nrow = int(bs[blocks].sum())
+    rowptr = np.zeros(nrow + 1, np.int64)
+    cap = nrow * (8 * (2 * a.nc + 4))
+    cols = np.zeros(cap, np.int32)
+    vals = np.zeros(cap)
+    L = lib()
+    L.or_jacobian_blocks.restype = C.c_int64
+    nnz = L.or_jacobian_blocks(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state),
+                               blocks.ctypes.data_as(C.c_void_p), C.c_int32(len(blocks)),
+                               rowptr.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+                               _p(vals), C.c_int64(cap))
+    assert nnz >= 0, nnz
+    return rowptr, cols[:nnz].copy(), vals[:nnz].copy()
+
+
 def jacobian_ref(fluid, grid, state, nthreads=1):
     """The reference's hand-written Jacobian (IteratorJacobian restated) as
     (rowptr, cols, vals) in the reference layout; nthreads OpenMP threads
@@ -335,6 +356,64 @@ def reduced_from_csr(fluid, grid, state, rp, cols, vals, F):
             r = r + Js @ t[d]
         rhs[a] = r
     return blocks, rhs, S, t, P, Sc
+
+
+def reduced_sampled(fluid, grid, state, sample):
+    """The reduced blocks and right-hand sides of the listed row blocks only,
+    as reduced_from_csr forms them, from the exact Jacobian rows of those
+    blocks and of their dependencies (jacobian_blocks): for grids whose full
+    oracle Jacobian is too slow to form.  Returns (blocks {(a, d)}, rhs {a},
+    cond {d: 2-norm condition number of d's local-row block J_ls})."""
+    import scipy.sparse as sp
+    nc, n = fluid.nc, grid.n
+    bs, off = layout(fluid, grid, state)
+    F = residual(fluid, grid, state)
+    labs = np.concatenate([state.label_m, state.label_f]).astype(int)
+    nx, ny = grid.nx, grid.ny
+
+    def deps(b):
+        k, c = divmod(b, n)
+        iz, rem = divmod(c, nx * ny)
+        iy, ix = divmod(rem, nx)
+        out = [b, (1 - k) * n + c]
+        for ok, o in ((ix + 1 < nx, c + 1), (ix > 0, c - 1), (iy + 1 < ny, c + nx), (iy > 0, c - nx),
+                      (iz + 1 < grid.nz, c + nx * ny), (iz > 0, c - nx * ny)):
+            if ok:
+                out.append(k * n + o)
+        return out
+
+    sample = [int(b) for b in sample]
+    need = sorted(set(sample) | {d for a in sample for d in deps(a)})
+    rp, cols, vals = jacobian_blocks(fluid, grid, state, need)
+    first = {}
+    r = 0
+    for b in need:
+        first[b] = r
+        r += int(bs[b])
+    N = int(off[-1])
+    J = sp.csr_matrix((vals, cols, rp), shape=(len(rp) - 1, N))
+    S, t, cond = {}, {}, {}
+    for d in need:
+        pr, se, mr, lr = reduced_split(nc, labs[d])
+        o = int(off[d])
+        Jl = J[[first[d] + i for i in lr]][:, o:o + int(bs[d])].toarray()
+        A_ls = Jl[:, se]
+        S[d] = np.linalg.solve(A_ls, Jl[:, pr])
+        t[d] = np.linalg.solve(A_ls, F[[o + i for i in lr]])
+        cond[d] = float(np.linalg.cond(A_ls))
+    blocks, rhs = {}, {}
+    for a in sample:
+        pr, se, mr, lr = reduced_split(nc, labs[a])
+        Jm = J[[first[a] + i for i in mr]].toarray()
+        rr = -F[[int(off[a]) + i for i in mr]]
+        for d in deps(a):
+            prd, sed, _, _ = reduced_split(nc, labs[d])
+            o = int(off[d])
+            Js = Jm[:, [o + i for i in sed]]
+            blocks[(a, d)] = Jm[:, [o + i for i in prd]] - Js @ S[d]
+            rr = rr + Js @ t[d]
+        rhs[a] = rr
+    return blocks, rhs, cond
 
 
 def pack(fluid, grid, state):

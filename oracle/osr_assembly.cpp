@@ -102,8 +102,11 @@ int or_props(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const 
 // continuum's cell; the water row on the P and Sw columns of those blocks;
 // fugacity and constraint rows on their own block only.  Columns sorted.
 // Returns the nnz, or -1 if cap is too small (rowptr is still filled).
-int64_t or_jacobian(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const or_state* s,
-                    int64_t* rowptr, int32_t* cols, double* vals, int64_t cap) {
+// The rows of the listed blocks (all blocks in order when blocks is NULL),
+// consecutively in list order: rowptr has one entry per listed row plus one.
+int64_t or_jacobian_blocks(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const or_state* s,
+                           const int32_t* blocks, int32_t nblocks, int64_t* rowptr, int32_t* cols, double* vals,
+                           int64_t cap) {
     Model M;
     build_model(M, f, fe, g);
     Sweep S;
@@ -132,12 +135,16 @@ int64_t or_jacobian(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g,
             liftp(S.P[k][c], PD[k][c], nc);
         }
     }
-    int64_t nnz = 0;
+    int64_t nnz = 0, rowno = 0;
     bool overflow = false;
     D rows[2 * NCM + 4];
     std::vector<double> blockJ;
-    for (int k = 0; k < 2; k++)
-        for (int c = 0; c < n; c++) {
+    const int32_t nlist = blocks ? nblocks : 2 * n;
+    for (int32_t bi = 0; bi < nlist; bi++) {
+        {
+            const int b = blocks ? blocks[bi] : bi;
+            if (b < 0 || b >= 2 * n) return -2;
+            const int k = b / n, c = b % n;
             const int iz = c / (M.nx * M.ny), rem = c % (M.nx * M.ny), iy = rem / M.nx, ix = rem % M.nx;
             const int bs = bsz[k * n + c];
             const int lab = S.V[k][c].label;
@@ -176,7 +183,7 @@ int64_t or_jacobian(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g,
                 // zero-porosity block: identity rows (Petsc_Functions.cpp:42051-42056)
                 for (int r = 0; r < bs; r++) {
                     const int64_t row = boff[k * n + c] + r;
-                    rowptr[row] = nnz;
+                    rowptr[rowno++] = nnz;
                     if (nnz < cap) { cols[nnz] = (int32_t)row; vals[nnz] = 1.0; }
                     else overflow = true;
                     nnz++;
@@ -184,8 +191,7 @@ int64_t or_jacobian(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g,
                 continue;
             }
             for (int r = 0; r < bs; r++) {
-                const int64_t row = boff[k * n + c] + r;
-                rowptr[row] = nnz;
+                rowptr[rowno++] = nnz;
                 const bool mass = (r >= r0 && r < r0 + nc);
                 const bool water = (r == r0 + nc);
                 for (size_t q = 0; q < deps.size(); q++) {
@@ -208,8 +214,14 @@ int64_t or_jacobian(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g,
                 }
             }
         }
-    rowptr[boff[2 * n]] = nnz;
+    }
+    rowptr[rowno] = nnz;
     return overflow ? -1 : nnz;
+}
+
+int64_t or_jacobian(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const or_state* s,
+                    int64_t* rowptr, int32_t* cols, double* vals, int64_t cap) {
+    return or_jacobian_blocks(f, fe, g, s, nullptr, 0, rowptr, cols, vals, cap);
 }
 
 // UpdateAll (PetscAUX.cpp:176-498): unknown vector X (reference layout) ->

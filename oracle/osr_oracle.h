@@ -155,6 +155,11 @@ int or_residual_scale(const or_fluid *f, const or_fluid_ext *fe, const or_grid *
 int or_props(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *out);
 int64_t or_jacobian(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s,
                     int64_t *rowptr, int32_t *cols, double *vals, int64_t cap);
+/* The rows of the listed blocks only (block = k n + c), consecutively in
+ * list order; rowptr [sum of the blocks' sizes + 1].  -2 for a bad block. */
+int64_t or_jacobian_blocks(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s,
+                           const int32_t *blocks, int32_t nblocks, int64_t *rowptr, int32_t *cols,
+                           double *vals, int64_t cap);
 /* IteratorJacobian (Petsc_Functions.cpp:6898-42073) restated: the reference's
  * hand-written Jacobian, its emitted columns (duplicates summed, sorted).
  * Returns nnz, -1 if cap is too small, -2 if a row emits a column outside

@@ -18,6 +18,7 @@ class FlashFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, octx, T, mode, P, z):
         out = rt.flash(octx, T, P.detach(), z.detach(), mode=mode)
+        ctx.P_dtype, ctx.P_device, ctx.z_dtype, ctx.z_device = P.dtype, P.device, z.dtype, z.device
         ctx.octx = octx
         ctx.T = T
         ctx.save_for_backward(P.detach(), z.detach(), out["K"], out["label"])
@@ -31,7 +32,10 @@ class FlashFn(torch.autograd.Function):
                                       xbar=None if xbar is None else xbar.contiguous(),
                                       ybar=None if ybar is None else ybar.contiguous(),
                                       Lbar=None if Lbar is None else Lbar.contiguous())
-        return None, None, None, Pbar, zbar
+        # the kernels work in float64 on the GPU: hand the gradients back in
+        # the dtype and on the device of the inputs
+        return None, None, None, Pbar.to(dtype=ctx.P_dtype, device=ctx.P_device), \
+            zbar.to(dtype=ctx.z_dtype, device=ctx.z_device)
 
 
 def flash(octx, T, P, z, mode="persistent"):

@@ -212,6 +212,9 @@ class NewtonBatch:
         with torch.cuda.graph(g):
             self._launch_all()
         self.graph = g
+        # the graph holds the context's flash, psat and phase-control
+        # workspaces: pin them until the graph is dropped
+        self.ctx.freeze(self)
         return g
 
     def replay(self):
@@ -220,6 +223,8 @@ class NewtonBatch:
         self.graph.replay()
 
     def close(self):
+        if self.graph is not None:
+            self.ctx.unfreeze(self)
         self.graph = None
         for it in self.items:
             it["asm"].close()

@@ -2439,6 +2439,7 @@ int osr_asm_workspace_size(osr_asm* a, size_t* bytes) {
 
 int osr_asm_set_workspace(osr_asm* a, void* ptr, size_t bytes) {
     if (!a) return OSR_E_ARG;
+    if (a->ctx && a->ctx->ws_frozen) return OSR_E_ARG;   // a captured graph may address the current one
     size_t off[4];
     const size_t need = ws_layout(a, off);
     if (ptr && (bytes < need || ((uintptr_t)ptr & 255))) return OSR_E_ARG;

@@ -122,6 +122,12 @@ int osr_psat_device(osr_ctx* ctx, int64_t n, const double* T, double T0, const d
     return osr::launch_psat(ctx, n, T, T0, P, kind, x, y, psat, xm, (hipStream_t)stream);
 }
 
+int osr_ctx_freeze_workspaces(osr_ctx* ctx, int frozen) {
+    if (!ctx || (frozen != 0 && frozen != 1)) return OSR_E_ARG;
+    ctx->ws_frozen = frozen;
+    return OSR_OK;
+}
+
 int osr_tune_set(osr_ctx* ctx, const char* key, double value) {
     if (!ctx || !key) return OSR_E_ARG;
     const std::string k(key);

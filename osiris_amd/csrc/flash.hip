@@ -825,6 +825,7 @@ static int ensure_park(osr_ctx* ctx, int nc, int64_t n, bool mixed) {
     const size_t words = park_words(nc, n, mixed);
     const size_t need = words * sizeof(double);
     if (need > ctx->park_bytes) {
+        if (ctx->ws_frozen) return OSR_E_SIZE;
         if (ctx->d_park) (void)hipFree(ctx->d_park);
         ctx->d_park = nullptr;
         ctx->park_bytes = 0;

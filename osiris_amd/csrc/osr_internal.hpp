@@ -35,6 +35,10 @@ struct osr_ctx {
     // Device address of this device's fixed park workspace (flash.hip
     // `scratch`), taken at context creation; used by osr_flash_device_noalloc.
     double* d_scratch = nullptr;
+    // while set, calls that would grow a workspace above (park store, psat
+    // list, phase-control queue) refuse with OSR_E_SIZE instead of freeing
+    // buffers a captured graph may still address (osr_ctx_freeze_workspaces)
+    int ws_frozen = 0;
     int mixed_warm_iters = 3;
     float mixed_tol32 = 1e-4f;
     int mixed_max32 = 150;

@@ -399,6 +399,7 @@ int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const doubl
     if (!T && !(T0 > 0)) return OSR_E_ARG;
     const size_t need = 256 + (size_t)n * sizeof(int32_t);
     if (need > ctx->plist_bytes) {
+        if (ctx->ws_frozen) return OSR_E_SIZE;
         if (ctx->d_plist) OSR_HIP(hipFree(ctx->d_plist));
         ctx->d_plist = nullptr;
         ctx->plist_bytes = 0;
@@ -493,6 +494,7 @@ int phase_ctrl(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* labe
     const size_t nc = ctx->nc;
     const size_t need = n * (sizeof(int8_t) + 2 * sizeof(double) + nc * sizeof(double)) + 1024;
     if (need > ctx->queue_bytes) {
+        if (ctx->ws_frozen) return OSR_E_SIZE;
         if (ctx->d_queue) hipFree(ctx->d_queue);
         ctx->d_queue = nullptr;
         ctx->queue_bytes = 0;

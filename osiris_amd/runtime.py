@@ -49,6 +49,7 @@ def lib():
         L.osr_version.restype = C.c_char_p
         L.osr_ctx_create.argtypes = [C.POINTER(_Fluid), C.POINTER(C.c_void_p)]
         L.osr_ctx_destroy.argtypes = [C.c_void_p]
+        L.osr_ctx_freeze_workspaces.argtypes = [C.c_void_p, C.c_int]
         L.osr_flash_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_double, C.c_void_p,
                                        C.c_void_p, C.c_double, C.POINTER(_FlashOut), C.c_int,
                                        C.c_void_p]
@@ -90,6 +91,21 @@ class Context:
         h = C.c_void_p()
         _check(lib().osr_ctx_create(C.byref(s), C.byref(h)), "osr_ctx_create")
         self.h = h
+
+    def freeze(self, owner):
+        """Pin the context's workspaces while `owner` (e.g. a captured
+        graph) holds their addresses (osr_ctx_freeze_workspaces); they stay
+        pinned until every owner has called unfreeze."""
+        holders = self.__dict__.setdefault("_ws_holders", set())
+        if not holders:
+            _check(lib().osr_ctx_freeze_workspaces(self.h, 1), "osr_ctx_freeze_workspaces")
+        holders.add(id(owner))
+
+    def unfreeze(self, owner):
+        holders = self.__dict__.get("_ws_holders", set())
+        holders.discard(id(owner))
+        if not holders and getattr(self, "h", None):
+            _check(lib().osr_ctx_freeze_workspaces(self.h, 0), "osr_ctx_freeze_workspaces")
 
     def close(self):
         if getattr(self, "h", None):
@@ -144,6 +160,27 @@ def alloc_flash_out(nc, n, device="cuda", full=True):
     return out
 
 
+_FLASH_OUT_SHAPE = dict(x="nc", y="nc", K="nc", L="n", label="n", Zl="n", Zg="n", so="n", sg="n",
+                        rho_l="n", rho_g="n", iters="n", rr_iters="n")
+_FLASH_OUT_DTYPE = dict(label="int8", iters="int32", rr_iters="int32")
+
+
+def _check_flash_out(out, nc, n):
+    """Every given output buffer has the shape, dtype and layout the
+    kernels write: [nc, n] or [n], contiguous, on the GPU."""
+    import torch
+    for k, v in out.items():
+        if k not in _FLASH_OUT_SHAPE:
+            raise ValueError(f"unknown flash output {k!r}")
+        if v is None:
+            continue
+        shape = (nc, n) if _FLASH_OUT_SHAPE[k] == "nc" else (n,)
+        dt = getattr(torch, _FLASH_OUT_DTYPE.get(k, "float64"))
+        if tuple(v.shape) != shape or v.dtype != dt or not v.is_cuda or not v.is_contiguous():
+            raise ValueError(f"flash output {k}: need a contiguous {dt} GPU tensor of shape {shape}, got "
+                             f"{tuple(v.shape)} {v.dtype} on {v.device}")
+
+
 def flash(ctx, T, P, z, sw=0.0, mode="persistent", out=None, stream=None, noalloc=False):
     """Flash n cells (SuccFalsh).  T: scalar or [n]; P: [n]; z: [nc, n].
     noalloc=True calls osr_flash_device_noalloc (fixed per-device workspace,
@@ -165,14 +202,18 @@ def flash(ctx, T, P, z, sw=0.0, mode="persistent", out=None, stream=None, noallo
             raise ValueError("T must be scalar or [n]")
     if out is None:
         out = alloc_flash_out(nc, n)
+    _check_flash_out(out, nc, n)
     fo = _FlashOut(*[(_ptr(out.get(k)).value if out.get(k) is not None else None)
                      for k, _ in _FlashOut._fields_])
     fn = lib().osr_flash_device_noalloc if noalloc else lib().osr_flash_device
     m = _MODES[mode] if isinstance(mode, str) else int(mode)
     rc = fn(ctx.h, n, _ptr(Tt), T0, _ptr(P), _ptr(z), float(sw), C.byref(fo), m, _stream_ptr(stream))
-    if rc == -4:
+    if rc == -4 and noalloc:
         raise OsrError(f"osr_flash_device_noalloc: {n} cells exceed the fixed workspace "
                        f"({lib().osr_flash_noalloc_max_cells(nc, m)} cells at nc={nc})")
+    if rc == -4:
+        raise OsrError(f"osr_flash_device: {n} cells need a larger park store while the context's "
+                       "workspaces are pinned (a captured graph holds them)")
     _check(rc, "osr_flash_device_noalloc" if noalloc else "osr_flash_device")
     return out
 
@@ -247,12 +288,19 @@ def psat(ctx, T, P, kind, x, y, stream=None):
     x = _as_dev(x, torch.float64)
     y = _as_dev(y, torch.float64)
     nc, n = x.shape
+    if nc != ctx.nc:
+        raise ValueError(f"x has {nc} components, context has {ctx.nc}")
     P = _as_dev(P, torch.float64)
     kind = _as_dev(kind, torch.int8)
+    if tuple(y.shape) != (nc, n) or tuple(P.shape) != (n,) or tuple(kind.shape) != (n,):
+        raise ValueError(f"psat: x {tuple(x.shape)}, y {tuple(y.shape)}, P {tuple(P.shape)} and kind "
+                         f"{tuple(kind.shape)} must agree on (nc, n) = ({nc}, {n})")
     if np.isscalar(T):
         Tt, T0 = None, float(T)
     else:
         Tt, T0 = _as_dev(T, torch.float64), 0.0
+        if tuple(Tt.shape) != (n,):
+            raise ValueError("T must be scalar or [n]")
     ps = torch.empty(n, dtype=torch.float64, device="cuda")
     xm = torch.empty(nc, n, dtype=torch.float64, device="cuda")
     _check(lib().osr_psat_device(ctx.h, n, _ptr(Tt), T0, _ptr(P), _ptr(kind), _ptr(x), _ptr(y),

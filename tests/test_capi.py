@@ -61,3 +61,8 @@ def test_noalloc_flash_limits():
     assert lib.osr_flash_noalloc_max_cells(7, 1) == -3
     assert lib.osr_flash_noalloc_max_cells(6, 3) == -1
     assert lib.osr_flash_device_noalloc(None, 1, None, 300.0, None, None, 0.0, None, 1, None) == -1
+
+
+def test_freeze_workspaces_rejects_bad_arguments():
+    lib = rt.lib()
+    assert lib.osr_ctx_freeze_workspaces(None, 1) == -1

@@ -67,6 +67,18 @@ def test_newton_iteration_graph(gpu, oracle, form):
     batch.run_eager()
     torch.cuda.synchronize()
     eager = _snapshot(batch)
+    # the eager run is finite where the reference's arithmetic is: every
+    # residual row, Jacobian entry, reduced block and right-hand side of
+    # these states, every flash x and L, y of the two-phase cells
+    for i in range(len(batch.items)):
+        assert torch.isfinite(eager[i]["F"]).all(), i
+        if form == "csr":
+            assert torch.isfinite(eager[i]["vals"]).all(), i
+        else:
+            assert torch.isfinite(eager[i]["red"][2]).all() and torch.isfinite(eager[i]["red"][3]).all(), i
+    fo = eager["flash"]
+    assert torch.isfinite(fo["x"]).all() and torch.isfinite(fo["L"]).all()
+    assert torch.isfinite(fo["y"][:, fo["label"] == 0]).all()
     batch.capture()
     for rep in range(3):
         for it in batch.items:            # clobber the outputs between replays
@@ -122,3 +134,31 @@ def test_newton_iteration_graph(gpu, oracle, form):
                 got = eager["flash"][k][..., off:off + n]
                 assert _equal(got.contiguous(), ref), k
             off += n
+
+
+def test_captured_graph_pins_the_context_workspaces(gpu, oracle):
+    """While a NewtonBatch graph exists, a call that would grow the
+    context's flash park store refuses (OSR_E_SIZE) instead of freeing the
+    buffer the graph addresses; closing the batch lifts it."""
+    from osiris_amd.batch import NewtonBatch
+    fl = synth.fluid_pseudo(6, synth.PR, 3)
+    cases = _cases(oracle, fl)[:2]
+    ctx = gpu.Context(fl)
+    batch = NewtonBatch(ctx, fl, cases, form="reduced", flash_mode="persistent")
+    batch.capture()
+    batch.replay()
+    torch.cuda.synchronize()
+    n = 4 * batch.ncells
+    g = synth.rng(3)
+    P = torch.as_tensor(g.uniform(8e6, 20e6, n)).cuda()
+    z = torch.as_tensor(g.dirichlet(np.full(fl.nc, 2.0), n).T.copy()).cuda()
+    print("frozen workspaces: flash of", n, "cells after a", batch.ncells, "-cell capture", flush=True)
+    with pytest.raises(gpu.OsrError, match="pinned"):
+        gpu.flash(ctx, cases[0][0].T, P, z, mode="persistent")
+    batch.replay()                      # the graph still runs on its workspaces
+    torch.cuda.synchronize()
+    batch.close()
+    out = gpu.flash(ctx, cases[0][0].T, P, z, mode="persistent")
+    torch.cuda.synchronize()
+    assert torch.isfinite(out["L"]).all()
+    ctx.close()
