@@ -232,6 +232,14 @@ def props(fluid, grid, state):
     return out
 
 
+def iftran(fluid, grid, state):
+    """IFK_Calc's face coefficients kA/h: [2 continua, 3 directions, n]."""
+    a = _Asm(fluid, grid, state)
+    out = np.zeros((2, 3, a.n))
+    lib().or_iftran(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), _p(out))
+    return out
+
+
 def jacobian(fluid, grid, state):
     """Reference-layout Jacobian as (rowptr, cols, vals)."""
     a = _Asm(fluid, grid, state)

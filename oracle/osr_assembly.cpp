@@ -96,6 +96,19 @@ int or_props(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const 
     return 0;
 }
 
+// IFK_Calc's face coefficients kA/h (Init_Calcs.cpp:21-42) of the model:
+// out [2][3][n] (continuum, direction, cell; the face to the +direction
+// neighbour, 0 where there is none).
+int or_iftran(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, double* out) {
+    Model M;
+    build_model(M, f, fe, g);
+    const int n = M.n;
+    for (int k = 0; k < 2; k++)
+        for (int d = 0; d < 3; d++)
+            for (int c = 0; c < n; c++) out[((int64_t)k * 3 + d) * n + c] = M.iftran[k][d][c];
+    return 0;
+}
+
 // Jacobian in the reference layout as CSR.  Row pattern: every row of a
 // block depends on its own block; component-balance and total rows on the
 // blocks of the six face neighbours (same continuum) and of the other

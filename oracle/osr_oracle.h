@@ -153,6 +153,8 @@ int or_residual(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, con
 /* Same rows with every term taken by magnitude: the scale of each row. */
 int or_residual_scale(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *F);
 int or_props(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *out);
+/* IFK_Calc's kA/h (Init_Calcs.cpp:21-42): out [2][3][n]. */
+int or_iftran(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, double *out);
 int64_t or_jacobian(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s,
                     int64_t *rowptr, int32_t *cols, double *vals, int64_t cap);
 /* The rows of the listed blocks only (block = k n + c), consecutively in
