@@ -516,6 +516,28 @@ def newton_leg(rt, synth, torch, args, log):
     }
 
 
+def workload_stats(cells, labels, iters, K):
+    """What the +/-15 %-of-Psat recipe yields under the reference's flash:
+    the fraction of cells below their saturation pressure, the two-phase
+    fraction overall and among those, and the near-critical fraction
+    (two-phase cells whose converged K-values all lie within
+    exp(+/-0.2))."""
+    has = cells.psat > 0
+    below = has & (cells.P < cells.psat)
+    two = labels == 0
+    lk = np.abs(np.log(np.where(K > 0, K, np.nan)))
+    near = two & (np.nanmax(np.where(np.isfinite(lk), lk, np.inf), axis=0) < 0.2)
+    return {"mean_ss_passes": float(iters.mean()), "max_ss_passes": int(iters.max()),
+            "frac_two_phase": float(two.mean()), "frac_capped_500": float((iters >= 500).mean()),
+            "frac_with_psat": float(has.mean()), "frac_below_psat": float(below.mean()),
+            "frac_two_phase_among_below_psat": float(two[below].mean()) if below.any() else 0.0,
+            "frac_near_critical": float(near.mean()),
+            "note": "SuccFalsh starts from Wilson K-values and has no stability test: cells below their "
+                    "bubble point whose Wilson Rachford-Rice has no root in (0, 1) stop after two passes "
+                    "at L = 1 (single-phase liquid), so about a third of the below-Psat cells end two-phase "
+                    "and the recipe's ~50 % target is not reached by the reference's own flash"}
+
+
 def flash_leg(rt, synth, torch, args):
     """The batched-flash workload: plain, persistent and mixed forms on 2M
     cells (8 components, PR), flop-model roofline of the persistent form."""
@@ -554,14 +576,14 @@ def flash_leg(rt, synth, torch, args):
     iters = out["iters"].cpu().numpy()
     rr = out["rr_iters"].cpu().numpy()
     labels = out["label"].cpu().numpy()
-    gpu_res = {k: out[k].cpu().numpy() for k in ("x", "y", "K", "L", "label", "iters")}
+    gpu_res = {k: out[k].cpu().numpy() for k in ("x", "y", "K", "L", "label", "iters", "rr_iters")}
     out_plain = rt.alloc_flash_out(NC, n)
     sec_plain, kern_plain = run("plain", out_plain, max(3, steps // 4), 1)
     tuned["mixed"] = tune.tune_flash(ctx, T, P, z, mode="mixed", reps=3)["config"]
     tune.apply(ctx, {"config": tuned["mixed"]})
     out_mixed = rt.alloc_flash_out(NC, n)
     sec_mixed, kern_mixed = run("mixed", out_mixed, steps, 1)
-    mixed_res = {k: out_mixed[k].cpu().numpy() for k in ("x", "y", "K", "L", "label", "iters")}
+    mixed_res = {k: out_mixed[k].cpu().numpy() for k in ("x", "y", "K", "L", "label", "iters", "rr_iters")}
 
     flops = flop_model(NC, iters, rr)
     achieved = flops / kern_persist / 1e12
@@ -575,11 +597,7 @@ def flash_leg(rt, synth, torch, args):
         "speedup_persistent_vs_plain": sec_plain / sec_persist,
         "speedup_mixed_vs_plain": sec_plain / sec_mixed,
         "tuned_knobs": tuned,
-        "workload_stats": {
-            "mean_ss_passes": float(iters.mean()), "max_ss_passes": int(iters.max()),
-            "frac_two_phase": float((labels == 0).mean()),
-            "frac_capped_500": float((iters >= 500).mean()),
-        },
+        "workload_stats": workload_stats(cells, labels, iters, gpu_res["K"]),
         "roofline": {
             "bound": "mfma",
             "bound_note": "FP64 vector ALU; MI355X FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s",
@@ -628,7 +646,7 @@ def srk_mixed_leg(rt, synth, torch, args):
         torch.cuda.synchronize()
         return float(np.mean([s_.elapsed_time(e) for s_, e in ev]))
 
-    keys = ("x", "y", "K", "L", "label", "iters")
+    keys = ("x", "y", "K", "L", "label", "iters", "rr_iters")
     res = {}
     ms = {}
     from osiris_amd import tune
@@ -672,9 +690,141 @@ def srk_mixed_leg(rt, synth, torch, args):
         "speedup_persistent_vs_plain": ms["plain"] / ms["persistent"],
         "tuned_knobs": tuned,
         "frac_two_phase": float((b["label"] == 0).mean()),
+        "workload_stats": workload_stats(cells, b["label"], b["iters"], b["K"]),
         "mixed_vs_gpu_fp64_every_cell": vs_fp64,
     }
     return detail, fl, cells, res
+
+
+# The flash parity bars of tests/test_gpu_flash.py (compare_flash), applied
+# by flash_bars to every cell of the full-size workloads.
+FLASH_TOL, FLASH_TOL_K, FLASH_TOL_SLOW = 1e-8, 1e-7, 1e-6
+FLASH_TOL_CAPPED, FLASH_TOL_CAPPED_MIXED = 1e-5, 5e-2
+FLASH_BRANCH_FRAC, FLASH_TOL_BRANCH = 2e-3, 1e-6
+
+
+SENS_EPS = (2.2e-16, 4.4e-16, 1e-15, 1e-14, 1e-13, 1e-12)
+
+
+def oracle_k_sensitivity(O, fl, cells):
+    """sens(c, target) -> (change, eps): the smallest relative perturbation
+    eps of cell c's pressure, or of one of its feed fractions, under which the
+    oracle's own K-values move by at least half of `target` (the GPU-oracle
+    difference), and that change; (largest change seen, None) if none of
+    SENS_EPS does -- how well K is determined by the FP64 inputs."""
+    def sens(c, target):
+        T, P, z = float(cells.T[c]), float(cells.P[c]), cells.z[:, c].copy()
+        K0 = O.flash_cell(fl, T, P, z)["K"]
+        worst = 0.0
+        for eps in SENS_EPS:
+            trials = [(P * (1 + eps), z), (P * (1 - eps), z)]
+            for i in range(len(z)):
+                z2 = z.copy()
+                z2[i] *= 1 + eps
+                trials.append((P, z2))
+            for Pp, zp in trials:
+                d = float(np.max(np.abs(O.flash_cell(fl, T, Pp, zp)["K"] / K0 - 1)))
+                worst = max(worst, d)
+                if d >= 0.5 * target:
+                    return d, eps
+        return worst, None
+    return sens
+
+
+def flash_bars(ref, res, mixed=False, sens=None):
+    """Every cell against the oracle under the flash tests' bars: labels
+    identical; x, y, L within 1e-8 (K 1e-7) on cells that stop within 100
+    passes, except RR-branch cells (at most 0.2 % of them, within 1e-6);
+    1e-6 on cells that need 100..499 passes; 1e-5 (mixed form 5e-2) where
+    either side hits the 500-pass cap.  Counts per category, the largest
+    error in each, the cells over their bar, and for the RR-branch cells how
+    many stop after a different number of Rachford-Rice steps than the
+    oracle."""
+    it = np.maximum(res["iters"], ref["iters"])
+    fast = it < 100
+    slow = (it >= 100) & (it < 500)
+    capped = it >= 500
+    tol_capped = FLASH_TOL_CAPPED_MIXED if mixed else FLASH_TOL_CAPPED
+    out = {"cells": int(it.size), "converged_lt_100_passes": int(fast.sum()), "slow_100_499": int(slow.sum()),
+           "capped_either_side": int(capped.sum()),
+           "label_mismatches": int((res["label"] != ref["label"]).sum())}
+    branch = np.zeros(it.size, bool)
+    bad = np.zeros(it.size, bool)
+    fields = {}
+    over_bar = 0
+    finite_mismatch = 0
+    for k in ("x", "y", "K", "L"):
+        fin = np.isfinite(ref[k])
+        finite_mismatch += int((fin != np.isfinite(res[k])).any(axis=0).sum() if fin.ndim == 2
+                               else (fin != np.isfinite(res[k])).sum())
+        e = np.where(fin, np.abs(res[k] - ref[k]) / np.maximum(np.abs(ref[k]), 1e-12), 0.0)
+        ec = e.max(axis=0) if e.ndim == 2 else e
+        tfast = FLASH_TOL_K if k == "K" else FLASH_TOL
+        over = fast & (ec > tfast)
+        branch |= over
+        f = {}
+        for name, mask, t in (("converged", fast & ~over, tfast), ("rr_branch", over, FLASH_TOL_BRANCH),
+                              ("slow", slow, FLASH_TOL_SLOW), ("capped", capped, tol_capped)):
+            m = float(ec[mask].max()) if mask.any() else 0.0
+            nbad = int((ec[mask] > t).sum())
+            bad |= mask & (ec > t)
+            over_bar += nbad
+            f[name] = {"max": m, "bar": t, "cells_over_bar": nbad}
+        fields[k] = f
+    out["fields"] = fields
+    nb = int(branch.sum())
+    allow = max(2, int(FLASH_BRANCH_FRAC * fast.sum()))
+    out["rr_branch_cells"] = nb
+    out["rr_branch_allowance"] = allow
+    if "rr_iters" in res and "rr_iters" in ref and nb:
+        out["rr_branch_cells_with_other_rr_count"] = int((res["rr_iters"][branch] != ref["rr_iters"][branch]).sum())
+    out["finite_pattern_mismatches"] = finite_mismatch
+    # the cells over their bar: passes and Rachford-Rice steps on both sides
+    idx = np.flatnonzero(bad)
+    eK = np.where(np.isfinite(ref["K"]), np.abs(res["K"] - ref["K"]) / np.maximum(np.abs(ref["K"]), 1e-12), 0.0)
+    eK = eK.max(axis=0)
+    out["cells_over_bar"] = int(idx.size)
+    if idx.size:
+        same_it = res["iters"][idx] == ref["iters"][idx]
+        out["over_bar_same_pass_count"] = int(same_it.sum())
+        if "rr_iters" in res and "rr_iters" in ref:
+            out["over_bar_same_rr_count"] = int((res["rr_iters"][idx] == ref["rr_iters"][idx]).sum())
+        # why each over-bar cell is over: its K-values are ill-conditioned
+        # in the reference algorithm itself (the oracle's K moves by at least
+        # half the difference under a relative input perturbation of at most
+        # 1e-12, the size of a few passes' accumulated rounding), the two
+        # sides stopped on different passes (the |dL| <= 1e-14 rule watches
+        # L only), or the oracle hit the 500-pass cap where the mixed form's
+        # Newton polish converged
+        kinds = {"ill_conditioned": 0, "other_stop_pass": 0, "mixed_vs_capped_oracle": 0, "unexplained": 0}
+        cells_out = []
+        for c in idx[:200]:
+            s, s_eps = sens(int(c), float(eK[c])) if sens is not None else (None, None)
+            if mixed and ref["iters"][c] >= 500:
+                kind = "mixed_vs_capped_oracle"
+            elif s_eps is not None:
+                kind = "ill_conditioned"
+            elif res["iters"][c] != ref["iters"][c]:
+                kind = "other_stop_pass"
+            else:
+                kind = "unexplained"
+            kinds[kind] += 1
+            if len(cells_out) < 12 or kind == "unexplained":
+                cells_out.append({"cell": int(c), "gpu_passes": int(res["iters"][c]),
+                                  "oracle_passes": int(ref["iters"][c]),
+                                  "gpu_rr": int(res["rr_iters"][c]) if "rr_iters" in res else None,
+                                  "oracle_rr": int(ref["rr_iters"][c]) if "rr_iters" in ref else None,
+                                  "K_rel": float(eK[c]), "L": float(ref["L"][c]),
+                                  "oracle_K_change": s, "at_input_perturbation": s_eps, "why": kind})
+        out["over_bar_why"] = kinds
+        out["over_bar_classified"] = int(min(idx.size, 200))
+        out["over_bar_cells"] = cells_out
+    out["meets_test_bars"] = bool(out["label_mismatches"] == 0 and over_bar == 0 and nb <= allow
+                                  and finite_mismatch == 0)
+    out["meets_test_bars_but_explained_cells"] = bool(
+        out["label_mismatches"] == 0 and nb <= allow and finite_mismatch == 0
+        and out.get("over_bar_why", {}).get("unexplained", 0) == 0 and idx.size <= 200)
+    return out
 
 
 def srk_oracle_parity(O, fl, cells, res, threads):
@@ -707,6 +857,8 @@ def srk_oracle_parity(O, fl, cells, res, threads):
                 worst = max(worst, float(np.max(np.abs(f_mine - f_ref))))
             lnf.append(worst)
         lnf = np.array(lnf)
+        out[mode + "_test_bars"] = flash_bars(ref, r, mixed=(mode == "mixed"),
+                                              sens=oracle_k_sensitivity(O, fl, cells))
         out[mode] = {
             "label_mismatches": int((r["label"] != ref["label"]).sum()),
             "cells_capped_either_side": int((~conv).sum()),
@@ -954,6 +1106,10 @@ def main():
                 "single_core_cells_per_s": m / cpu1_s, "single_core_sample": f"first {m} cells, 1 thread"}
             result.setdefault("parity_vs_oracle", {})["flash_persistent_fp64"] = parity(gpu_res)
             result["parity_vs_oracle"]["flash_mixed_fp32_fp64"] = parity(mixed_res)
+            sens = oracle_k_sensitivity(O, fl, cells)
+            result["parity_vs_oracle"]["flash_persistent_fp64_test_bars"] = flash_bars(ref, gpu_res, sens=sens)
+            result["parity_vs_oracle"]["flash_mixed_fp32_fp64_test_bars"] = flash_bars(ref, mixed_res, mixed=True,
+                                                                                   sens=sens)
         if not args.no_srk:
             result.setdefault("parity_vs_oracle", {})["flash_mixed_srk12"] = srk_oracle_parity(
                 O, sfl, scells, sgpu, threads)

@@ -46,7 +46,7 @@ class _Fluid(C.Structure):
 class _FlashOut(C.Structure):
     _fields_ = [("L", C.c_double), ("Zl", C.c_double), ("Zg", C.c_double),
                 ("so", C.c_double), ("sg", C.c_double), ("rho_l", C.c_double),
-                ("rho_g", C.c_double), ("label", C.c_int), ("iters", C.c_int)]
+                ("rho_g", C.c_double), ("label", C.c_int), ("iters", C.c_int), ("rr_iters", C.c_int)]
 
 
 def _p(a):
@@ -104,7 +104,7 @@ def flash_cell(fluid, T, P, z, sw=0.0):
     lib().or_flash(f.ref, C.c_double(T), C.c_double(P), _p(z), C.c_double(sw),
                    _p(x), _p(y), _p(K), C.byref(o))
     return dict(x=x, y=y, K=K, L=o.L, Zl=o.Zl, Zg=o.Zg, so=o.so, sg=o.sg,
-                rho_l=o.rho_l, rho_g=o.rho_g, label=o.label, iters=o.iters)
+                rho_l=o.rho_l, rho_g=o.rho_g, label=o.label, iters=o.iters, rr_iters=o.rr_iters)
 
 
 def flash_batch(fluid, T, P, z, sw=0.0, nthreads=0):
@@ -117,12 +117,12 @@ def flash_batch(fluid, T, P, z, sw=0.0, nthreads=0):
     T0 = float(T) if np.isscalar(T) else 0.0
     out = dict(x=np.zeros((f.nc, n)), y=np.zeros((f.nc, n)), K=np.zeros((f.nc, n)),
                L=np.zeros(n), label=np.zeros(n, np.int8), Zl=np.zeros(n),
-               Zg=np.zeros(n), iters=np.zeros(n, np.int32))
-    lib().or_flash_batch(
+               Zg=np.zeros(n), iters=np.zeros(n, np.int32), rr_iters=np.zeros(n, np.int32))
+    lib().or_flash_batch_rr(
         f.ref, C.c_int64(n), _p(Tarr), C.c_double(T0), _p(P), _p(z), C.c_double(sw),
         _p(out["x"]), _p(out["y"]), _p(out["K"]), _p(out["L"]),
         out["label"].ctypes.data_as(C.c_void_p), _p(out["Zl"]), _p(out["Zg"]),
-        out["iters"].ctypes.data_as(C.c_void_p), C.c_int(nthreads))
+        out["iters"].ctypes.data_as(C.c_void_p), out["rr_iters"].ctypes.data_as(C.c_void_p), C.c_int(nthreads))
     return out
 
 

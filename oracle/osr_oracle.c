@@ -161,7 +161,7 @@ void or_flash(const or_fluid *f, double T, double P, const double *z, double sw,
     const int nc = f->nc;
     double a[NCMAX], b[NCMAX], tl[NCMAX], tg[NCMAX];
     double d1, d2, L = 0, Lb = 10, ESum, Zl = 0, Zg = 0;
-    int k = 0;
+    int k = 0, rr = 0;
     or_eos_ab(f, T, a, b);
     eos_d12(f->eos, &d1, &d2);
     for (int i = 0; i < nc; i++) K[i] = wilson(f, i, T, P);
@@ -182,6 +182,7 @@ void or_flash(const or_fluid *f, double T, double P, const double *z, double sw,
             L -= step;
             j++;
         } while (step * step > OR_NEWTON_TOL && j < OR_MAX_FLASH_N_ITR);
+        rr += j;
 
         if (L > 1) L = 1;                                /* :608-615 */
         else if (L < 0) L = 0;
@@ -238,12 +239,21 @@ void or_flash(const or_fluid *f, double T, double P, const double *z, double sw,
     o->Zl = Zl;
     o->Zg = Zg;
     o->iters = k;
+    o->rr_iters = rr;
 }
 
 void or_flash_batch(const or_fluid *f, int64_t n, const double *T, double T0,
                     const double *P, const double *z, double sw,
                     double *x, double *y, double *K, double *L, int8_t *label,
                     double *Zl, double *Zg, int32_t *iters, int nthreads)
+{
+    or_flash_batch_rr(f, n, T, T0, P, z, sw, x, y, K, L, label, Zl, Zg, iters, NULL, nthreads);
+}
+
+void or_flash_batch_rr(const or_fluid *f, int64_t n, const double *T, double T0,
+                       const double *P, const double *z, double sw,
+                       double *x, double *y, double *K, double *L, int8_t *label,
+                       double *Zl, double *Zg, int32_t *iters, int32_t *rr, int nthreads)
 {
     const int nc = f->nc;
 #ifdef _OPENMP
@@ -265,6 +275,7 @@ void or_flash_batch(const or_fluid *f, int64_t n, const double *T, double T0,
         if (Zl) Zl[c] = o.Zl;
         if (Zg) Zg[c] = o.Zg;
         if (iters) iters[c] = o.iters;
+        if (rr) rr[c] = o.rr_iters;
     }
     (void)nthreads;
 }

@@ -65,6 +65,7 @@ typedef struct {
     double L, Zl, Zg, so, sg, rho_l, rho_g;
     int label;   /* 1 liquid, 0 two-phase, -1 gas (Functions.h:327) */
     int iters;   /* outer successive-substitution iterations */
+    int rr_iters;/* Rachford-Rice Newton iterations over all passes */
 } or_flash_out;
 
 /* SuccFalsh (Init_Calcs.cpp:571-738) for one cell. */
@@ -77,6 +78,11 @@ void or_flash_batch(const or_fluid *f, int64_t n, const double *T, double T0,
                     const double *P, const double *z, double sw,
                     double *x, double *y, double *K, double *L, int8_t *label,
                     double *Zl, double *Zg, int32_t *iters, int nthreads);
+/* The same, with the Rachford-Rice iteration count of each cell (rr [n]). */
+void or_flash_batch_rr(const or_fluid *f, int64_t n, const double *T, double T0,
+                       const double *P, const double *z, double sw,
+                       double *x, double *y, double *K, double *L, int8_t *label,
+                       double *Zl, double *Zg, int32_t *iters, int32_t *rr, int nthreads);
 
 /* Bubl_Succ (Minor_NR.cpp:491-871).  x: liquid composition (comp[..][0]),
  * ystale: the cell's stored vapour composition (comp[..][1]), which the
