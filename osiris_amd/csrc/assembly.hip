@@ -727,8 +727,9 @@ template <int NC, class T>
 __device__ __forceinline__ void loc_piece(const AsmArgs& A, const VarsT<NC, T>& v, const PropsT<NC, T>& p,
                                           RowAcc<NC, T>& R);
 
+// the accumulation of the balance rows alone
 template <int NC, class T>
-__device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const Geo& G, const VarsT<NC, T>& v,
+__device__ __forceinline__ void acc_piece(const AsmArgs& A, int k, int c, const Geo& G, const VarsT<NC, T>& v,
                                           const PropsT<NC, T>& p, RowAcc<NC, T>& R) {
     const AsmConsts& K = *A.K;
     const int n = A.n;
@@ -747,6 +748,12 @@ __device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const 
         const double pr = pre ? pre[(int64_t)NC * n + c] : dval(acc);
         R.bal[NC] += f * (acc - pr);
     }
+}
+
+template <int NC, class T>
+__device__ __forceinline__ void own_piece(const AsmArgs& A, int k, int c, const Geo& G, const VarsT<NC, T>& v,
+                                          const PropsT<NC, T>& p, RowAcc<NC, T>& R) {
+    acc_piece<NC, T>(A, k, c, G, v, p, R);
     loc_piece<NC, T>(A, v, p, R);
 }
 
@@ -1865,8 +1872,10 @@ __device__ __forceinline__ int dep_rank(const AsmArgs& A, int k, const Geo& G, i
 // One lane per (block a, direction q): every reduced column that a's
 // direction q feeds.  The lane seeds a along q once (a's variables and
 // property derivatives in registers) and forms
-//   * a's own block: accumulation, local rows, its six faces and its
-//     transfer term;
+//   * a's own block: accumulation, its six faces and its transfer term
+//     (its local rows are k_red_loc's: forming them here too would be dead
+//     work that the compiler does not remove -- 2.0 k instructions and most
+//     of the register spill at nc = 10);
 //   * the blocks of its face neighbours b, (b, a): each face's terms of b's
 //     balance rows, taken from the same face evaluation (face_piece MIRROR);
 //   * the other continuum's block of the same cell: its transfer term.
@@ -1913,7 +1922,7 @@ __global__ __launch_bounds__(64) void k_red_own(AsmArgs A, const double* dred, c
     PropsT<NC, D1> sp;
     load_props_dual<NC>(A, dred, blk, q, sp, NP + 1, JacShape<NC>::RF);
     RowAcc<NC, D1> R;
-    if (live) own_piece<NC, D1>(A, k, c, G, sv, sp, R);
+    if (live) acc_piece<NC, D1>(A, k, c, G, sv, sp, R);
     for (int f = 0; f < 6; f++) {
         const FaceGeo FG = face_geo(A, G, f);
         const int o = FG.o;
