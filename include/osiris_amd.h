@@ -36,7 +36,7 @@ extern "C" {
  * from the deck: fluidProp[Nc][COMP_PROPS] (Functions.h:96-111, filled at
  * Data_Input.cpp:572-951) and bic[Nc][Nc] = 1 - k_ij (Data_Input.cpp:977-990). */
 typedef struct osr_fluid {
-    int32_t nc;              /* 1..16 */
+    int32_t nc;              /* 2..12 compiled (osr_supported_nc); 1..16 else OSR_E_NC */
     int32_t eos;             /* OSR_EOS_PR or OSR_EOS_SRK (deck MODEL keyword) */
     const double *Tc;        /* [nc] critical temperature, K */
     const double *Pc;        /* [nc] critical pressure, Pa */
@@ -523,7 +523,8 @@ int osr_rst_write_device(osr_asm *a, const osr_state_dev *s, const double *props
 int osr_rst_read_device(osr_asm *a, const uint8_t *img, osr_state_rw_dev *s, double *phase_p, osr_rst_scalars *sc,
                         void *stream);
 
-/* Library version string and the component counts compiled in. */
+/* Library version string and the component counts compiled in: every nc
+ * from 2 to 12 (osr_supported_nc returns 1 for those, 0 otherwise). */
 const char *osr_version(void);
 int osr_supported_nc(int32_t nc);
 

@@ -2261,17 +2261,18 @@ int expand_nc(osr_asm* a, const osr_state_dev* s, const double* S, const double*
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
-#define OSR_NC_SWITCH(nc, CALL)      \
-    switch (nc) {                    \
-        case 2: return CALL(2);      \
-        case 3: return CALL(3);      \
-        case 4: return CALL(4);      \
-        case 5: return CALL(5);      \
-        case 6: return CALL(6);      \
-        case 8: return CALL(8);      \
-        case 10: return CALL(10);    \
-        case 12: return CALL(12);    \
-        default: return OSR_E_NC;    \
+// switch over the compiled component counts; the call site defines CALL(NC)
+#define OSR_NC_RET_(N) case N: return CALL(N);
+#define OSR_NC_RC_(N) case N: rc = CALL(N); break;
+#define OSR_NC_SWITCH(nc, CALL_)                   \
+    switch (nc) {                                  \
+        OSR_NC_LIST(OSR_NC_RET_)                   \
+        default: return OSR_E_NC;                  \
+    }
+#define OSR_NC_SWITCH_RC(nc)                       \
+    switch (nc) {                                  \
+        OSR_NC_LIST(OSR_NC_RC_)                    \
+        default: return OSR_E_NC;                  \
     }
 
 }  // namespace
@@ -2509,17 +2510,7 @@ int osr_jacobian_rowptr_device(osr_asm* a, const osr_state_dev* s, int64_t* rowp
     if (!rowptr) return OSR_E_ARG;
     hipStream_t st = (hipStream_t)stream;
 #define CALL(NC) rowptr_nc<NC>(a, s, rowptr, a->nrows, st)
-    switch (a->nc) {
-        case 2: rc = CALL(2); break;
-        case 3: rc = CALL(3); break;
-        case 4: rc = CALL(4); break;
-        case 5: rc = CALL(5); break;
-        case 6: rc = CALL(6); break;
-        case 8: rc = CALL(8); break;
-        case 10: rc = CALL(10); break;
-        case 12: rc = CALL(12); break;
-        default: return OSR_E_NC;
-    }
+    OSR_NC_SWITCH_RC(a->nc)
 #undef CALL
     if (rc) return rc;
     if (nnz) {
@@ -2545,17 +2536,9 @@ int osr_jacobian_ref_rowptr_device(osr_asm* a, const osr_state_dev* s, int64_t* 
     if (rc) return rc;
     if (!rowptr) return OSR_E_ARG;
     hipStream_t st = (hipStream_t)stream;
-    switch (a->nc) {
-        case 2: rc = rj_rowptr_nc<2>(a, s, rowptr, a->nrows, st); break;
-        case 3: rc = rj_rowptr_nc<3>(a, s, rowptr, a->nrows, st); break;
-        case 4: rc = rj_rowptr_nc<4>(a, s, rowptr, a->nrows, st); break;
-        case 5: rc = rj_rowptr_nc<5>(a, s, rowptr, a->nrows, st); break;
-        case 6: rc = rj_rowptr_nc<6>(a, s, rowptr, a->nrows, st); break;
-        case 8: rc = rj_rowptr_nc<8>(a, s, rowptr, a->nrows, st); break;
-        case 10: rc = rj_rowptr_nc<10>(a, s, rowptr, a->nrows, st); break;
-        case 12: rc = rj_rowptr_nc<12>(a, s, rowptr, a->nrows, st); break;
-        default: return OSR_E_NC;
-    }
+#define CALL(NC) rj_rowptr_nc<NC>(a, s, rowptr, a->nrows, st)
+    OSR_NC_SWITCH_RC(a->nc)
+#undef CALL
     if (rc) return rc;
     if (nnz) {
         OSR_HIP(hipMemcpyAsync(nnz, rowptr + a->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -2642,17 +2625,7 @@ int osr_props_device(osr_asm* a, const osr_state_dev* s, double* out, void* stre
     hipStream_t st = (hipStream_t)stream;
     osr::AsmArgs A = make_args(a, s);
 #define CALL(NC) run_props<NC>(a, A, st)
-    switch (a->nc) {
-        case 2: rc = CALL(2); break;
-        case 3: rc = CALL(3); break;
-        case 4: rc = CALL(4); break;
-        case 5: rc = CALL(5); break;
-        case 6: rc = CALL(6); break;
-        case 8: rc = CALL(8); break;
-        case 10: rc = CALL(10); break;
-        case 12: rc = CALL(12); break;
-        default: return OSR_E_NC;
-    }
+    OSR_NC_SWITCH_RC(a->nc)
 #undef CALL
     if (rc) return rc;
     const size_t nf = 16 + 2 * (size_t)a->nc;
@@ -2773,7 +2746,7 @@ int osr_unpack_device(osr_asm* a, const osr_state_mut_dev* s, const double* X, v
 #define UNPACK(NC) case NC: hipLaunchKernelGGL(osr::k_unpack<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, a->n, a->boff, \
         s->label_m, s->label_f, s->Po_m, s->Po_f, s->x_m, s->y_m, s->x_f, s->y_f, s->S_m, s->S_f, s->pcoeff, X); break;
     switch (a->nc) {
-        UNPACK(2) UNPACK(3) UNPACK(4) UNPACK(5) UNPACK(6) UNPACK(8) UNPACK(10) UNPACK(12)
+        OSR_NC_LIST(UNPACK)
         default: return OSR_E_NC;
     }
 #undef UNPACK
@@ -2790,7 +2763,7 @@ int osr_pack_device(osr_asm* a, const osr_state_dev* s, double* X, void* stream)
 #define PACK(NC) case NC: hipLaunchKernelGGL(osr::k_pack<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, a->n, a->boff, \
         s->label_m, s->label_f, s->Po_m, s->Po_f, s->x_m, s->y_m, s->x_f, s->y_f, s->S_m, s->S_f, s->pcoeff, X); break;
     switch (a->nc) {
-        PACK(2) PACK(3) PACK(4) PACK(5) PACK(6) PACK(8) PACK(10) PACK(12)
+        OSR_NC_LIST(PACK)
         default: return OSR_E_NC;
     }
 #undef PACK
@@ -2823,17 +2796,9 @@ int asm_sweep(osr_asm* a, const osr_state_dev* s, double* rho_out, hipStream_t s
     if (rc0) return rc0;
     osr::AsmArgs A = make_args(a, s);
     int rc = OSR_E_NC;
-    switch (a->nc) {
-        case 2: rc = run_props<2>(a, A, st); break;
-        case 3: rc = run_props<3>(a, A, st); break;
-        case 4: rc = run_props<4>(a, A, st); break;
-        case 5: rc = run_props<5>(a, A, st); break;
-        case 6: rc = run_props<6>(a, A, st); break;
-        case 8: rc = run_props<8>(a, A, st); break;
-        case 10: rc = run_props<10>(a, A, st); break;
-        case 12: rc = run_props<12>(a, A, st); break;
-        default: break;
-    }
+#define CALL(NC) run_props<NC>(a, A, st)
+    OSR_NC_SWITCH_RC(a->nc)
+#undef CALL
     if (rc) return rc;
     return rho_out ? asm_take_rho(a, rho_out, st) : OSR_OK;
 }

@@ -911,7 +911,10 @@ static int launch_nc(osr_ctx* ctx, const FlashArgs& a, int mode, bool noalloc, h
 
 bool nc_supported(int nc) {
     switch (nc) {
-        case 2: case 3: case 4: case 5: case 6: case 8: case 10: case 12: return true;
+#define CASE(NC) case NC:
+        OSR_NC_LIST(CASE)
+#undef CASE
+        return true;
         default: return false;
     }
 }
@@ -947,14 +950,9 @@ int launch_flash(osr_ctx* ctx, int64_t n, const double* T, double T0, const doub
     a.tol32 = ctx->mixed_tol32;
     a.max32 = ctx->mixed_max32;
     switch (ctx->nc) {
-        case 2: return launch_nc<2>(ctx, a, mode, noalloc, stream);
-        case 3: return launch_nc<3>(ctx, a, mode, noalloc, stream);
-        case 4: return launch_nc<4>(ctx, a, mode, noalloc, stream);
-        case 5: return launch_nc<5>(ctx, a, mode, noalloc, stream);
-        case 6: return launch_nc<6>(ctx, a, mode, noalloc, stream);
-        case 8: return launch_nc<8>(ctx, a, mode, noalloc, stream);
-        case 10: return launch_nc<10>(ctx, a, mode, noalloc, stream);
-        case 12: return launch_nc<12>(ctx, a, mode, noalloc, stream);
+#define CASE(NC) case NC: return launch_nc<NC>(ctx, a, mode, noalloc, stream);
+        OSR_NC_LIST(CASE)
+#undef CASE
         default: return OSR_E_NC;
     }
 }

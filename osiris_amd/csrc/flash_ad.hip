@@ -345,14 +345,9 @@ static int launch_ad(osr_ctx* ctx, const AdArgs& a, bool adjoint, hipStream_t st
 
 static int dispatch_ad(osr_ctx* ctx, const AdArgs& a, bool adjoint, hipStream_t st) {
     switch (ctx->nc) {
-        case 2: return launch_ad<2>(ctx, a, adjoint, st);
-        case 3: return launch_ad<3>(ctx, a, adjoint, st);
-        case 4: return launch_ad<4>(ctx, a, adjoint, st);
-        case 5: return launch_ad<5>(ctx, a, adjoint, st);
-        case 6: return launch_ad<6>(ctx, a, adjoint, st);
-        case 8: return launch_ad<8>(ctx, a, adjoint, st);
-        case 10: return launch_ad<10>(ctx, a, adjoint, st);
-        case 12: return launch_ad<12>(ctx, a, adjoint, st);
+#define CASE(NC) case NC: return launch_ad<NC>(ctx, a, adjoint, st);
+        OSR_NC_LIST(CASE)
+#undef CASE
         default: return OSR_E_NC;
     }
 }

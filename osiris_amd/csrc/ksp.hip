@@ -377,17 +377,12 @@ int ksp_solve_bs(osr_ksp* k, const double* b, double* x, const osr_ksp_opts* o, 
     return OSR_OK;
 }
 
-#define OSR_BS_SWITCH(bs, CALL)   \
-    switch (bs) {                 \
-        case 3: return CALL(3);   \
-        case 4: return CALL(4);   \
-        case 5: return CALL(5);   \
-        case 6: return CALL(6);   \
-        case 7: return CALL(7);   \
-        case 9: return CALL(9);   \
-        case 11: return CALL(11); \
-        case 13: return CALL(13); \
-        default: return OSR_E_ARG; \
+// block sizes nc + 1 of the compiled component counts
+#define OSR_BS_RET_(NC) case NC + 1: return CALL(NC + 1);
+#define OSR_BS_SWITCH(bs, CALL_)          \
+    switch (bs) {                         \
+        OSR_NC_LIST(OSR_BS_RET_)          \
+        default: return OSR_E_ARG;        \
     }
 
 }  // namespace
@@ -396,7 +391,7 @@ extern "C" {
 
 int osr_ksp_create(int32_t bs, int64_t nbrows, osr_ksp** out) {
     if (!out || nbrows < 1) return OSR_E_ARG;
-    if (bs != 3 && bs != 4 && bs != 5 && bs != 6 && bs != 7 && bs != 9 && bs != 11 && bs != 13) return OSR_E_ARG;
+    if (!osr::nc_supported(bs - 1)) return OSR_E_ARG;
     osr_ksp* k = new (std::nothrow) osr_ksp();
     if (!k) return OSR_E_ARG;
     k->bs = bs;

@@ -44,6 +44,12 @@ struct osr_ctx {
     int mixed_max32 = 150;
 };
 
+// Component counts compiled in: every nc from 2 to 12.  Each kernel keeps a
+// cell's per-component vectors in registers with its component loops
+// unrolled, so nc is a template argument; OSR_NC_LIST(X) expands X(nc) for
+// each compiled count and every dispatch switch is built from it.
+#define OSR_NC_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+
 #define OSR_HIP(call)                                   \
     do {                                                \
         hipError_t e_ = (call);                         \

@@ -413,14 +413,9 @@ int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const doubl
                        psat, xm, list, count);
     PsatArgs a{n, T, T0, P, kind, x, y, psat, xm, list, count};
     switch (ctx->nc) {
-        case 2: return launch_psat_nc<2>(ctx, a, stream);
-        case 3: return launch_psat_nc<3>(ctx, a, stream);
-        case 4: return launch_psat_nc<4>(ctx, a, stream);
-        case 5: return launch_psat_nc<5>(ctx, a, stream);
-        case 6: return launch_psat_nc<6>(ctx, a, stream);
-        case 8: return launch_psat_nc<8>(ctx, a, stream);
-        case 10: return launch_psat_nc<10>(ctx, a, stream);
-        case 12: return launch_psat_nc<12>(ctx, a, stream);
+#define CASE(NC) case NC: return launch_psat_nc<NC>(ctx, a, stream);
+        OSR_NC_LIST(CASE)
+#undef CASE
         default: return OSR_E_NC;
     }
 }
@@ -512,7 +507,7 @@ int phase_ctrl(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* labe
     if (rc) return rc;
     switch (ctx->nc) {
 #define APPLY(NC) case NC: hipLaunchKernelGGL(k_phase_apply<NC>, dim3(g), dim3(256), 0, st, n, label, Po, Pg, x, y, S, ps, xm); break;
-        APPLY(2) APPLY(3) APPLY(4) APPLY(5) APPLY(6) APPLY(8) APPLY(10) APPLY(12)
+        OSR_NC_LIST(APPLY)
 #undef APPLY
         default: return OSR_E_NC;
     }
