@@ -354,7 +354,10 @@ int osr_props_device(osr_asm *a, const osr_state_dev *s, double *out, void *stre
  * derived; else 256-thread blocks per CU of the persistent flash grid), and
  * for OSR_FLASH_MIXED "flash.mixed_warm_iters" (FP64 passes before the FP32
  * stage, 1..100), "flash.mixed_tol32" (FP32 stage stop on max |K'/K - 1|) and
- * "flash.mixed_max32" (FP32 stage pass cap). */
+ * "flash.mixed_max32" (FP32 stage pass cap); for the reduced system straight
+ * from the state "reduced.local_rows" (1: the local rows from closed-form
+ * cubic-EOS derivatives and eliminated in one lane group per block -- the
+ * default; 0: forward mode per (block, unknown), then the elimination). */
 int osr_tune_set(osr_ctx *ctx, const char *key, double value);
 
 /* Tangent of the flash at a converged state (no counterpart in the

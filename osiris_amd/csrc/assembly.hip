@@ -29,6 +29,7 @@
 #include <type_traits>
 
 #include "dual_dev.hpp"
+#include "grp_dev.hpp"
 #include "osr_internal.hpp"
 
 namespace osr {
@@ -1787,6 +1788,202 @@ __global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, 
     group_write_st<NC>(col, lab, nl, j, bs, Sb, tb);
 }
 
+// The local rows of a block and their elimination in one group of RED_GROUP
+// lanes, the local-row Jacobian formed from the closed-form derivatives of
+// the cubic EOS instead of forward mode.  Same values as k_red_loc +
+// k_red_local<NC, false> up to rounding; no per-column workspace.
+//
+// Fugacity row i of a two-phase block is R_i = fL_i - fG_i with
+// f_i = w_i P phi_i(w, P) of the phase's composition w (x or y) and pressure
+// P (Po; Pg = Po + pcg(Sg) bift(x, y) / IFTBASE in the matrix, Pg = Po in
+// the fracture), ln phi_i = t1_i (Z - 1) - ln(Z - B) + DD EE_i FF with
+// t1_i = b_i / Cb, EE_i = 2 t_i / Ca - t1_i, DD = A / (B (d1 - d2)),
+// FF = ln((Z + d2 B) / (Z + d1 B)), t_i = sum_m a_im w_m, Ca = sum w_i t_i,
+// Cb = sum b_i w_i, A = Ca P / (RT)^2, B = Cb P / RT.  Along composition m of
+// the same phase (dCa = sa_m sum_j u_j (bic_mj + bic_jm), dCb = b_m,
+// dt_i = a_im, dZ = Z_A dA + Z_B dB from the cubic's implicit function):
+//   d ln phi_i = b_i alpha_m + EE_i beta_m + t_i gamma_m + a_im kappa - dlz_m
+//   alpha_m = (DD FF - (Z - 1)) b_m / Cb^2 + dZ_m / Cb,
+//   beta_m  = dDD_m FF + DD dFF_m,  gamma_m = -2 DD FF dCa_m / Ca^2,
+//   kappa   = 2 DD FF / Ca,  dlz_m = (dZ_m - dB_m) / (Z - B),
+// and df_i = delta_im P phi_i + f_i d ln phi_i; along P (unit dP)
+// g_i = df_i/dP = w_i phi_i + f_i (t1_i dZ_P - dlz_P + DD EE_i dFF_P).  The
+// matrix gas pressure adds -g_i^G dPg to every column that moves Pg
+// (compositions through the parachor IFT, Sg through the Pc table).
+//
+// Lanes [0, 16) hold the liquid phase and [16, 32) the vapour, lane i of a
+// half component i: the mixing row, the per-component quantities f_i,
+// f_i EE_i, f_i t_i, g_i.  Then lane j holds column j of the local rows
+// (InitGuess2's unknown order: x, y, P, Sw, So, Sg), gathering the row
+// quantities by cross-lane reads, lane 31 the residual column, and the group
+// eliminates as k_red_local does.
+template <int NC>
+__global__ __launch_bounds__(256) void k_red_locel(AsmArgs A, const double* F, double* S, double* t) {
+    constexpr int NLM = NC + 3, NP = NC + 1;
+    static_assert(NC <= 16, "one component per lane of a 16-lane half");
+    static_assert(2 * NC + 4 < RED_GROUP, "a block's unknowns and its residual column fit one lane group");
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = (int)(item / RED_GROUP);
+    const int j = (int)(item % RED_GROUP);
+    const int n = A.n;
+    if (blk >= 2 * n) return;                    // whole groups leave together
+    const int gbase = (int)(threadIdx.x & 63) & ~(RED_GROUP - 1);
+    const int k = blk / n, c = blk % n;
+    const int lab = A.lab[k][c];
+    const int bs = lab == 0 ? 2 * NC + 4 : NC + 3;
+    double* Sb = S + (int64_t)blk * NLM * NP;
+    double* tb = t + (int64_t)blk * NLM;
+    double col[NLM];
+#pragma unroll
+    for (int r = 0; r < NLM; r++) col[r] = 0.0;
+    if (A.phi[k * n + c] < 1e-10) {
+        group_write_st<NC>(col, lab, 0, j, bs, Sb, tb);
+        return;
+    }
+    const int nl = nloc(lab, NC);
+    const int64_t rb = A.boff[blk];
+    if (j == RED_GROUP - 1) {
+#pragma unroll
+        for (int r = 0; r < NLM; r++)
+            if (r < nl) col[r] = F[rb + loc_row(lab, NC, r)];
+    }
+    if (lab != 0) {
+        // sum of the present phase's compositions - 1, sum S - 1 (columns:
+        // compositions, P, Sw, So|Sg)
+        if (j < NC) col[0] = 1.0;
+        else if (j == NC + 1 || j == NC + 2) col[1] = 1.0;
+        group_eliminate<NC, 2>(col, lab, gbase);
+        group_write_st<NC>(col, lab, nl, j, bs, Sb, tb);
+        return;
+    }
+    const AsmConsts& K = *A.K;
+    const double rt = RGAS * K.T;
+    const double dd = K.d1 - K.d2;
+    const bool srk = K.eos == OSR_EOS_SRK;
+    const int64_t s2 = 2 * (int64_t)n;
+    const double* tab = A.props + blk;
+    // ---- per half: phase h, component i
+    const int h = (j >> 4) & 1;
+    const int i = j & 15;
+    const bool ival = i < NC;
+    const int ic = ival ? i : 0;
+    const double* wsrc = (h == 0 ? A.x[k] : A.y[k]) + c;
+    double u[NC];
+    double Cb = 0.0, spar = 0.0;
+#pragma unroll
+    for (int m = 0; m < NC; m++) {
+        const double w = wsrc[(int64_t)m * n];
+        u[m] = w * K.sa[m];
+        Cb += w * K.b[m];
+        spar += w * K.par[m];
+    }
+    double srow = 0.0, scol = 0.0;
+#pragma unroll
+    for (int m = 0; m < NC; m++) {
+        srow += u[m] * K.bic[ic * NC + m];
+        scol += u[m] * K.bic[m * NC + ic];
+    }
+    const double ti = srow * K.sa[ic];
+    const double dCai = (srow + scol) * K.sa[ic];   // dCa / dw_i
+    const double Ca = half_sum<16>(ival ? u[ic] * srow : 0.0);
+    const double wi = ival ? wsrc[(int64_t)ic * n] : 0.0;
+    const double P = tab[(2 + h) * s2];
+    const double Z = P / (tab[(5 + h) * s2] * rt);   // the sweep's root
+    const double Bb = Cb * P / rt, Aa = Ca * P / (rt * rt);
+    const double a1 = srk ? -1.0 : Bb - 1.0;
+    const double a2 = srk ? Aa - Bb * Bb - Bb : Aa - 3.0 * Bb * Bb - 2.0 * Bb;
+    const double ifz = -1.0 / ((3.0 * Z + 2.0 * a1) * Z + a2);
+    const double ZA = (Z - Bb) * ifz;
+    const double ZB = (srk ? -(2.0 * Bb + 1.0) * Z - Aa : (Z - 6.0 * Bb - 2.0) * Z - Aa + (3.0 * Bb + 2.0) * Bb) * ifz;
+    const double DD = Aa / (Bb * dd);
+    const double Zd2 = Z + K.d2 * Bb, Zd1 = Z + K.d1 * Bb;
+    const double FF = log(Zd2 / Zd1);
+    const double ZmB = Z - Bb;
+    const double lz = log(ZmB);
+    const double iCa = 1.0 / Ca, iCb = 1.0 / Cb;
+    const double t1 = K.b[ic] * iCb;
+    const double EE = 2.0 * ti * iCa - t1;
+    const double phi = exp(t1 * (Z - 1.0) - lz + DD * EE * FF);
+    const double fi = wi * P * phi;
+    // along P: dA = Ca / RT^2, dB = Cb / RT
+    const double dAP = Ca / (rt * rt), dBP = Cb / rt;
+    const double dZP = ZA * dAP + ZB * dBP;
+    const double dFFP = (dZP + K.d2 * dBP) / Zd2 - (dZP + K.d1 * dBP) / Zd1;
+    const double dlzP = (dZP - dBP) / ZmB;
+    const double gi = wi * phi + fi * (t1 * dZP - dlzP + DD * EE * dFFP);
+    // ---- per column
+    const int hc = j < NC ? 0 : 1;                 // phase of a composition column
+    const int m = j < NC ? j : (j < 2 * NC ? j - NC : 0);
+    const int own = gbase + 16 * hc + m;           // owner lane of the column's component
+    const int same = gbase + 16 * hc + i;          // this lane's place in the column's half
+    const double cZ = __shfl(Z, same, 64), cB = __shfl(Bb, same, 64), cCa = __shfl(Ca, same, 64);
+    const double cCb = __shfl(Cb, same, 64), cP = __shfl(P, same, 64);
+    const double cZA = __shfl(ZA, same, 64), cZB = __shfl(ZB, same, 64), cDD = __shfl(DD, same, 64);
+    const double cFF = __shfl(FF, same, 64);
+    const double cdCa = __shfl(dCai, own, 64), cPphi = __shfl(P * phi, own, 64);
+    const double sparL = __shfl(spar, gbase, 64), sparG = __shfl(spar, gbase + 16, 64);
+    // composition column m of phase hc
+    const double bm = K.b[m];
+    const double dA = cdCa * cP / (rt * rt), dB = bm * cP / rt;
+    const double dZ = cZA * dA + cZB * dB;
+    const double cZd2 = cZ + K.d2 * cB, cZd1 = cZ + K.d1 * cB;
+    const double dFF = (dZ + K.d2 * dB) / cZd2 - (dZ + K.d1 * dB) / cZd1;
+    const double dlz = (dZ - dB) / (cZ - cB);
+    const double icCa = 1.0 / cCa, icCb = 1.0 / cCb;
+    const double dDD = (cdCa - cCa * bm * icCb) * icCb / (rt * dd);
+    const double alpha = (cDD * cFF - (cZ - 1.0)) * bm * icCb * icCb + dZ * icCb;
+    const double beta = dDD * cFF + cDD * dFF;
+    const double gamma = -2.0 * cDD * cFF * cdCa * icCa * icCa;
+    const double kappa = 2.0 * cDD * cFF * icCa;
+    // the matrix gas pressure's parachor-IFT and Pc-table dependence
+    double dPgx = 0.0, dPgy = 0.0, dPgS = 0.0;
+    if (k == 0) {
+        const double bift = tab[4 * s2];
+        int r = 0;
+        const double sg = A.S[0][2 * (int64_t)n + c];
+        while (r < K.nsgt && !(sg < K.sgt[4 * r])) r++;
+        const double slope = (r == 0 || r == K.nsgt)
+                                 ? 0.0
+                                 : (K.sgt[4 * r + 3] - K.sgt[4 * (r - 1) + 3]) / (K.sgt[4 * r] - K.sgt[4 * (r - 1)]);
+        if (bift < MISCIBLEIFT) {
+            const double rpo = A.rho_prev ? A.rho_prev[c] : 0.0, rpg = A.rho_prev ? A.rho_prev[n + c] : 0.0;
+            const double ps = sparL * rpo - sparG * rpg;
+            const double cpl = tab[15 * s2] / IFTBASECASE * 3.88 * bift / ps;
+            dPgx = cpl * K.par[m] * rpo;
+            dPgy = -cpl * K.par[m] * rpg;
+            dPgS = slope * bift / IFTBASECASE;
+        } else {
+            dPgS = slope;
+        }
+    }
+    const bool xcol = j < NC, ycol = j >= NC && j < 2 * NC;
+    const double dPgc = xcol ? dPgx : dPgy;
+#pragma unroll
+    for (int r = 0; r < NC; r++) {
+        const int src = gbase + 16 * hc + r;
+        const double f_r = __shfl(fi, src, 64);
+        const double fe_r = __shfl(fi * EE, src, 64);
+        const double ft_r = __shfl(fi * ti, src, 64);
+        const double gL = __shfl(gi, gbase + r, 64);
+        const double gG = __shfl(gi, gbase + 16 + r, 64);
+        if (xcol || ycol) {
+            const double a_rm = K.sa[r] * K.sa[m] * K.bic[r * NC + m];
+            double dfr = f_r * (K.b[r] * alpha - dlz + kappa * a_rm) + fe_r * beta + ft_r * gamma;
+            if (r == m) dfr += cPphi;
+            col[r] = (xcol ? dfr : -dfr) - gG * dPgc;
+        } else if (j == 2 * NC) {
+            col[r] = A.pcoeff * (gL - gG);
+        } else if (j == 2 * NC + 3) {
+            col[r] = -gG * dPgS;
+        }
+    }
+    if (xcol) col[NC] = 1.0;
+    if (ycol) col[NC + 1] = 1.0;
+    if (j >= 2 * NC + 1 && j <= 2 * NC + 3) col[NC + 2] = 1.0;
+    group_eliminate<NC, NC + 3>(col, lab, gbase);
+    group_write_st<NC>(col, lab, nl, j, bs, Sb, tb);
+}
+
 // one lane per (block, direction q <= NP): the property sweep in forward
 // mode along the reduced direction q of the block (v_q, or w for q = NP;
 // the phase roots from the property table), written with the direction's
@@ -2231,7 +2428,8 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     constexpr int NF = osr::JacShape<NC>::NF, MAXC = osr::JacShape<NC>::MAXC;
     constexpr int NP = NC + 1;
     const int64_t nb = 2 * (int64_t)a->n;
-    if (!a->wloc) OSR_HIP(hipMalloc((void**)&a->wloc, (size_t)nb * MAXC * (NC + 3) * sizeof(double)));
+    const bool fwd_local = a->ctx->red_local_form == 0;
+    if (fwd_local && !a->wloc) OSR_HIP(hipMalloc((void**)&a->wloc, (size_t)nb * MAXC * (NC + 3) * sizeof(double)));
     if (!a->dred)
         OSR_HIP(hipMalloc((void**)&a->dred, (size_t)nb * osr::JacShape<NC>::RF * (NP + 1) * sizeof(double)));
     if (!a->rpart) OSR_HIP(hipMalloc((void**)&a->rpart, (size_t)nb * 8 * NP * sizeof(double)));
@@ -2239,9 +2437,17 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     const unsigned gq = (unsigned)((nb * (NP + 1) + 255) / 256);
     // the one-wave-per-SIMD kernels run in one-wave workgroups, so each SIMD
     // takes its next wave as soon as its own finishes
-    hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3((unsigned)((nb * MAXC + 63) / 64)), dim3(64), 0, st, A, a->wloc);
-    hipLaunchKernelGGL((osr::k_red_local<NC, false>), dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256),
-                       0, st, A, (const double*)a->wloc, F, (const int64_t*)nullptr, (const double*)nullptr, S, t);
+    if (fwd_local) {
+        // the local rows in forward mode, one lane per (block, unknown), then
+        // their elimination
+        hipLaunchKernelGGL(osr::k_red_loc<NC>, dim3((unsigned)((nb * MAXC + 63) / 64)), dim3(64), 0, st, A, a->wloc);
+        hipLaunchKernelGGL((osr::k_red_local<NC, false>), dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)),
+                           dim3(256), 0, st, A, (const double*)a->wloc, F, (const int64_t*)nullptr,
+                           (const double*)nullptr, S, t);
+    } else {
+        hipLaunchKernelGGL(osr::k_red_locel<NC>, dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256), 0,
+                           st, A, F, S, t);
+    }
     hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)S, (const double*)t,
                        a->dred);
     hipLaunchKernelGGL(osr::k_red_own<NC>, dim3((unsigned)((nb * (NP + 1) + 63) / 64)), dim3(64), 0, st, A,

@@ -39,6 +39,10 @@ struct osr_ctx {
     // list, phase-control queue) refuse with OSR_E_SIZE instead of freeing
     // buffers a captured graph may still address (osr_ctx_freeze_workspaces)
     int ws_frozen = 0;
+    // local rows of the reduced system straight from the state: 1 = closed-form
+    // EOS derivatives and elimination in one lane group per block
+    // (k_red_locel), 0 = forward mode per (block, unknown) then elimination
+    int red_local_form = 1;
     int mixed_warm_iters = 3;
     float mixed_tol32 = 1e-4f;
     int mixed_max32 = 150;
