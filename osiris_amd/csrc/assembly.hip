@@ -623,26 +623,37 @@ __device__ __forceinline__ void face_piece(const AsmArgs& A, int k, const Geo& G
         const int absent = ph == 0 ? -1 : 1;
         const double Cf = vo.lab == absent ? 0.0 : 1.0;
         const double dcs = vs.lab == absent ? 0.0 : 1.0;
+        // neither cell has the phase: every coefficient is 0 and the rows
+        // (finite compositions) are unchanged
+        if (Cf == 0 && dcs == 0) continue;
         const T AA = (po.ro[ph] * (dn * Cf) + ps.ro[ph] * (dl * dcs)) *
                      ((vo.S[1 + ph] * (dn * Cf) + vs.S[1 + ph] * (dl * dcs)) / (dn + dl));
         const T BB = (tn * dn * Cf) / po.porpor + (ts * dl * dcs) / ps.porpor;
         const T MN = AA / BB * NN;
+        if (zerophi) {
+            // a zero-porosity cell takes D = 0: the coefficient is MN * 0,
+            // NaN where MN is (0/0 for a neighbour lacking the phase), as in
+            // the reference
+            const T Dt = MN * T(0.0);
+#pragma unroll
+            for (int i = 0; i < NC - 1; i++) {
+                const TS xs = ph == 0 ? vs.x[i] : vs.y[i];
+                const TO xn = ph == 0 ? vo.x[i] : vo.y[i];
+                R.bal[i] -= Dt * (xn - xs);
+                if constexpr (MIRROR) mb[i] += Dt * (xn - xs);
+            }
+            continue;
+        }
 #pragma unroll
         for (int i = 0; i < NC - 1; i++) {
             const TS xs = ph == 0 ? vs.x[i] : vs.y[i];
             const TO xn = ph == 0 ? vo.x[i] : vo.y[i];
-            T Dup = (xn > xs) ? T(po.D[ph][i]): T(ps.D[ph][i]);
-            if (zerophi) Dup = T(0.0);
-            T Dt = MN * Dup;
-            if (Cf == 0 && dcs == 0) Dt = T(0.0);
-            R.bal[i] -= Dt * (xn - xs);
+            const T Dup = (xn > xs) ? T(po.D[ph][i]): T(ps.D[ph][i]);
+            R.bal[i] -= (MN * Dup) * (xn - xs);
             if constexpr (MIRROR) {
                 // the neighbour's row: upstream by its own comparison
-                T Dum = (xs > xn) ? T(ps.D[ph][i]): T(po.D[ph][i]);
-                if (zerophi) Dum = T(0.0);
-                T Dtm = MN * Dum;
-                if (Cf == 0 && dcs == 0) Dtm = T(0.0);
-                mb[i] += Dtm * (xn - xs);
+                const T Dum = (xs > xn) ? T(ps.D[ph][i]): T(po.D[ph][i]);
+                mb[i] += (MN * Dum) * (xn - xs);
             }
         }
     }
@@ -2055,15 +2066,26 @@ __global__ __launch_bounds__(256, 2) void k_red_dprops(AsmArgs A, const double* 
     for (int j = 0; j < 3; j++) o[(NF + 2 * NC + 1 + j) * (NP + 1)] = sv.S[j].d[0];
 }
 
-// rank of dependency d among the row block's existing dependencies in
-// block order (its position in the reduced block row)
-__device__ __forceinline__ int dep_rank(const AsmArgs& A, int k, const Geo& G, int c, int d) {
-    int rank = 0;
-    for (int sl = 0; sl < 8; sl++) {
-        const int e = dep_block(A, k, G, c, sl);
-        if (e >= 0 && e < d) rank++;
+// Position of a dependency in the reduced block row of row block b
+// (continuum kb, cell coordinates ix, iy, iz): the row's existing
+// dependencies in block order are the other continuum's block first when
+// kb = 1, then b's own continuum by cell offset -nx*ny, -nx, -1, 0, +1, +nx,
+// +nx*ny (faces 5, 3, 1, b itself, 0, 2, 4; a face exists inside the grid),
+// then the other continuum's block when kb = 0.  slot: 0 b itself, 1 + g
+// the neighbour across b's face g, 7 the other continuum.
+__device__ __forceinline__ int red_rank(const AsmArgs& A, int kb, int ix, int iy, int iz, int slot) {
+    const int e5 = iz > 0, e3 = iy > 0, e1 = ix > 0;
+    const int e0 = ix < A.nx - 1, e2 = iy < A.ny - 1, e4 = iz < A.nz - 1;
+    if (slot == 7) return kb == 1 ? 0 : 1 + e0 + e1 + e2 + e3 + e4 + e5;
+    switch (slot) {
+        case 6: return kb;                                     // face 5
+        case 4: return kb + e5;                                // face 3
+        case 2: return kb + e5 + e3;                           // face 1
+        case 0: return kb + e5 + e3 + e1;                      // b itself
+        case 1: return kb + e5 + e3 + e1 + 1;                  // face 0
+        case 3: return kb + e5 + e3 + e1 + 1 + e0;             // face 2
+        default: return kb + e5 + e3 + e1 + 1 + e0 + e2;       // face 4
     }
-    return rank;
 }
 
 // One lane per (block a, direction q): every reduced column that a's
@@ -2100,13 +2122,15 @@ __global__ __launch_bounds__(64) void k_red_own(AsmArgs A, const double* dred, c
     }
     // column q of reduced block (row block b, column block blk), or the
     // slot partial of b's right-hand side
-    auto emit = [&](int b, int slot, int kb, const Geo& Gb, int cb, const double (&col)[NP]) {
+    // (slot: b's dependency slot of a, bslot: the same for b's right-hand side;
+    // ix, iy, iz: b's cell)
+    auto emit = [&](int b, int slot, int kb, int ix, int iy, int iz, const double (&col)[NP]) {
         if (q == NP) {
 #pragma unroll
             for (int i = 0; i < NP; i++) rpart[((int64_t)slot * NP + i) * nb + b] = col[i];
             return;
         }
-        const int64_t bz = browptr[b] + dep_rank(A, kb, Gb, cb, blk);
+        const int64_t bz = browptr[b] + red_rank(A, kb, ix, iy, iz, slot);
         if (q == 0) bcols[bz] = blk;
 #pragma unroll
         for (int i = 0; i < NP; i++) bvals[bz * NP * NP + i * NP + q] = col[i];
@@ -2158,7 +2182,10 @@ __global__ __launch_bounds__(64) void k_red_own(AsmArgs A, const double* dred, c
             double col[NP];
 #pragma unroll
             for (int i = 0; i < NP; i++) col[i] = mb[i].d[0];
-            emit(k * n + o, (f ^ 1) + 1, k, geom(A, o), o, col);   // a sits across b's face f ^ 1
+            // a sits across b's face f ^ 1
+            const int sx = f == 0 ? 1 : (f == 1 ? -1 : 0), sy = f == 2 ? 1 : (f == 3 ? -1 : 0);
+            const int sz = f == 4 ? 1 : (f == 5 ? -1 : 0);
+            emit(k * n + o, (f ^ 1) + 1, k, G.ix + sx, G.iy + sy, G.iz + sz, col);
         }
     }
     {
@@ -2167,25 +2194,42 @@ __global__ __launch_bounds__(64) void k_red_own(AsmArgs A, const double* dred, c
         PropsT<NC, double> px;
         load_vars<NC>(A, kx, c, vx);
         load_props<NC>(A, kx, c, px);
-        if (live) {
-            if (k == 0) transfer_piece<NC, D1, double>(A, 0, G, c, sv, sp, vx, px, R);
-            else transfer_piece<NC, double, D1>(A, 1, G, c, vx, px, sv, sp, R);
+        const bool xlive = A.phi[kx * n + c] >= 1e-10;
+        // With equal labels in the two continua the transfer term of the
+        // other continuum's rows is the negated term of this one's, operation
+        // for operation (the convective part changes only its sign, the
+        // diffusive cases (1,1), (-1,-1), (0,0) add and subtract the same
+        // products), so it is formed once; the mixed-state cases differ.
+        const bool same = vx.lab == sv.lab;
+        RowAcc<NC, D1> Rt;
+        if (live || (xlive && same)) {
+            if (k == 0) transfer_piece<NC, D1, double>(A, 0, G, c, sv, sp, vx, px, Rt);
+            else transfer_piece<NC, double, D1>(A, 1, G, c, vx, px, sv, sp, Rt);
         }
-        if (A.phi[kx * n + c] >= 1e-10) {
-            RowAcc<NC, D1> Rx;
-            if (k == 0) transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, vx, px, Rx);
-            else transfer_piece<NC, double, D1>(A, 0, G, c, vx, px, sv, sp, Rx);
-            double col[NP];
+        if (live) {
 #pragma unroll
-            for (int i = 0; i < NP; i++) col[i] = Rx.bal[i].d[0];
-            emit(kx * n + c, 7, kx, G, c, col);
+            for (int i = 0; i < NP; i++) R.bal[i] += Rt.bal[i];
+        }
+        if (xlive) {
+            double col[NP];
+            if (same) {
+#pragma unroll
+                for (int i = 0; i < NP; i++) col[i] = -Rt.bal[i].d[0];
+            } else {
+                RowAcc<NC, D1> Rx;
+                if (k == 0) transfer_piece<NC, D1, double>(A, 1, G, c, sv, sp, vx, px, Rx);
+                else transfer_piece<NC, double, D1>(A, 0, G, c, vx, px, sv, sp, Rx);
+#pragma unroll
+                for (int i = 0; i < NP; i++) col[i] = Rx.bal[i].d[0];
+            }
+            emit(kx * n + c, 7, kx, G.ix, G.iy, G.iz, col);
         }
     }
     if (!live) return;
     double col[NP];
 #pragma unroll
     for (int i = 0; i < NP; i++) col[i] = R.bal[i].d[0];
-    emit(blk, 0, k, G, c, col);
+    emit(blk, 0, k, G.ix, G.iy, G.iz, col);
 }
 
 template <int NC>
