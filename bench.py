@@ -501,7 +501,7 @@ def newton_leg(rt, synth, torch, args, log):
         "roofline": {
             "bound": "hbm",
             "kernel": "reduced-system assembly of the 256x256x32 grid (osr_jacobian_bsr_reduced_direct_device: "
-                      "k_props, k_red_loc, k_red_local, k_red_dprops, k_red_own, k_red_rhs)",
+                      "k_props, k_red_locel, k_red_dprops, k_red_own, k_red_rhs)",
             "achieved": red_bytes / (ms_red * 1e-3) / 1e9,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

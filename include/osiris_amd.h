@@ -357,7 +357,9 @@ int osr_props_device(osr_asm *a, const osr_state_dev *s, double *out, void *stre
  * "flash.mixed_max32" (FP32 stage pass cap); for the reduced system straight
  * from the state "reduced.local_rows" (1: the local rows from closed-form
  * cubic-EOS derivatives and eliminated in one lane group per block -- the
- * default; 0: forward mode per (block, unknown), then the elimination). */
+ * default; 0: forward mode per (block, unknown), then the elimination) and
+ * "assembly.xcd_remap" (1: each of the chip's 8 XCDs takes a contiguous
+ * eighth of the reduced-system kernels' workgroups; 0: dispatch order). */
 int osr_tune_set(osr_ctx *ctx, const char *key, double value);
 
 /* Tangent of the flash at a converged state (no counterpart in the

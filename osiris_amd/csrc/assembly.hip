@@ -83,7 +83,22 @@ struct AsmArgs {
     // derived
     double* props;           // [NF][2n]
     const int64_t* boff;     // [2n+1]
+    int xcd;                 // 1: XCD-contiguous workgroup order (wg_index)
 };
+
+// Logical workgroup index.  The dispatcher deals workgroups round-robin to
+// the chip's 8 XCDs (workgroup i to XCD i mod 8), each with its own L2, so
+// consecutive workgroups -- neighbouring cells of the stencil -- land in
+// different L2s.  With A.xcd set, XCD x walks one contiguous eighth of the
+// grid instead: hardware workgroup i = 8 j + x is logical workgroup
+// start_x + j, start_x = x q + min(x, r) for g = 8 q + r workgroups (a
+// bijection on [0, g)).
+__device__ __forceinline__ int64_t wg_index(const AsmArgs& A) {
+    const unsigned b = blockIdx.x;
+    if (!A.xcd) return b;
+    const unsigned g = gridDim.x, q = g >> 3, r = g & 7, x = b & 7, j = b >> 3;
+    return (int64_t)x * q + (x < r ? x : r) + j;
+}
 
 __device__ __forceinline__ int nfields(int nc) { return 16 + 2 * nc; }
 
@@ -1829,11 +1844,11 @@ __global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, 
 // quantities by cross-lane reads, lane 31 the residual column, and the group
 // eliminates as k_red_local does.
 template <int NC>
-__global__ __launch_bounds__(256) void k_red_locel(AsmArgs A, const double* F, double* S, double* t) {
+__global__ __launch_bounds__(256, 4) void k_red_locel(AsmArgs A, const double* F, double* S, double* t) {
     constexpr int NLM = NC + 3, NP = NC + 1;
     static_assert(NC <= 16, "one component per lane of a 16-lane half");
     static_assert(2 * NC + 4 < RED_GROUP, "a block's unknowns and its residual column fit one lane group");
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t item = wg_index(A) * blockDim.x + threadIdx.x;
     const int blk = (int)(item / RED_GROUP);
     const int j = (int)(item % RED_GROUP);
     const int n = A.n;
@@ -2005,7 +2020,7 @@ template <int NC>
 __global__ __launch_bounds__(256, 2) void k_red_dprops(AsmArgs A, const double* S, const double* t, double* dred) {
     using D1 = DualV<1>;
     constexpr int NF = JacShape<NC>::NF, NLM = NC + 3, NP = NC + 1;
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t item = wg_index(A) * blockDim.x + threadIdx.x;
     if (item >= (int64_t)2 * A.n * (NP + 1)) return;
     const int q = (int)(item % (NP + 1));
     const int blk = (int)(item / (NP + 1));
@@ -2108,7 +2123,7 @@ __global__ __launch_bounds__(64) void k_red_own(AsmArgs A, const double* dred, c
                                                  int32_t* bcols, double* bvals, double* rpart) {
     using D1 = DualV<1>;
     constexpr int NP = NC + 1;
-    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t item = wg_index(A) * blockDim.x + threadIdx.x;
     if (item >= (int64_t)2 * A.n * (NP + 1)) return;
     const int q = (int)(item % (NP + 1));
     const int blk = (int)(item / (NP + 1));
@@ -2335,6 +2350,7 @@ osr::AsmArgs make_args(osr_asm* a, const osr_state_dev* s) {
     A.pcoeff = s->pcoeff;
     A.props = a->props;
     A.boff = a->boff;
+    A.xcd = a->ctx->xcd_remap;
     return A;
 }
 

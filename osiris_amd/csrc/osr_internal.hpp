@@ -43,6 +43,9 @@ struct osr_ctx {
     // EOS derivatives and elimination in one lane group per block
     // (k_red_locel), 0 = forward mode per (block, unknown) then elimination
     int red_local_form = 1;
+    // reduced-system kernels: 1 = each XCD takes a contiguous eighth of the
+    // grid's workgroups (neighbouring cells share an L2), 0 = dispatch order
+    int xcd_remap = 0;
     int mixed_warm_iters = 3;
     float mixed_tol32 = 1e-4f;
     int mixed_max32 = 150;
