@@ -289,6 +289,18 @@ def newton_leg(rt, synth, torch, args, log):
     # launch knobs chosen by the autotuner on this grid's cells
     from osiris_amd import tune
     gb, sb = big
+    # launch knobs of the reduced-system assembly, tuned on this grid's state
+    # before any reduced-form graph is captured (a captured launch keeps its shape)
+    asm_t = rt.Assembler(ctx, fl, gb)
+    d_t = asm_t.device_state(sb)
+    asm_t.layout(d_t)
+    F_t = asm_t.residual(d_t)
+    tuned_a = tune.tune_assembly(ctx, asm_t, d_t, F_t, reps=3)
+    tune.apply(ctx, tuned_a)
+    asm_t.close()
+    del asm_t, d_t, F_t
+    torch.cuda.empty_cache()
+    log(f"newton: reduced-assembly knobs {tuned_a['config']} ({tuned_a['ms']:.1f} ms)")
     Pb = torch.tensor(np.concatenate([sb.Po_m, sb.Po_f]), device="cuda")
     zb = torch.tensor(np.concatenate([sb.z_m, sb.z_f], axis=1), device="cuda")
     tuned = tune.tune_flash(ctx, gb.T, Pb, zb, mode="mixed", reps=3)
@@ -466,6 +478,9 @@ def newton_leg(rt, synth, torch, args, log):
                                                  "flash_stages": {k: int(v) for k, v in mixed_stats.items()}},
                 "speedup_analytic_vs_plain": ms_plain / ms_mixed_big,
                 "reduced_assembly_ms": ms_red,
+                "reduced_assembly_tuned_knobs": tuned_a["config"],
+                "reduced_assembly_tuner_trials": [{"knobs": {k: v for k, v in c.items() if c[k] != tuned_a["config"][k]},
+                                                   "ms": round(m, 2)} for c, m in tuned_a["trials"]],
                 "linear_solve": {
                     "method": "BiCGStab + block Jacobi on the reduced system (osr_ksp_*)",
                     "setup_ms": ms_ksp_setup, "singular_diagonal_blocks": nsing,

@@ -359,7 +359,9 @@ int osr_props_device(osr_asm *a, const osr_state_dev *s, double *out, void *stre
  * cubic-EOS derivatives and eliminated in one lane group per block -- the
  * default; 0: forward mode per (block, unknown), then the elimination) and
  * "assembly.xcd_remap" (1: each of the chip's 8 XCDs takes a contiguous
- * eighth of the reduced-system kernels' workgroups; 0: dispatch order). */
+ * eighth of the reduced-system kernels' workgroups; 0: dispatch order),
+ * "reduced.dprops_wg" and "reduced.own_wg" (64, 128 or 256: workgroup
+ * sizes of the reduced system's direction sweep and balance-row kernels). */
 int osr_tune_set(osr_ctx *ctx, const char *key, double value);
 
 /* Tangent of the flash at a converged state (no counterpart in the

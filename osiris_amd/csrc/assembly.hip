@@ -2119,7 +2119,7 @@ __device__ __forceinline__ int red_rank(const AsmArgs& A, int kb, int ix, int iy
 // identity (Petsc_Functions.cpp:42051-42056) and still feeds its
 // neighbours; no row of a zero-porosity block is written here otherwise.
 template <int NC>
-__global__ __launch_bounds__(64) void k_red_own(AsmArgs A, const double* dred, const int64_t* browptr,
+__global__ __launch_bounds__(256) void k_red_own(AsmArgs A, const double* dred, const int64_t* browptr,
                                                  int32_t* bcols, double* bvals, double* rpart) {
     using D1 = DualV<1>;
     constexpr int NP = NC + 1;
@@ -2494,7 +2494,10 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
         OSR_HIP(hipMalloc((void**)&a->dred, (size_t)nb * osr::JacShape<NC>::RF * (NP + 1) * sizeof(double)));
     if (!a->rpart) OSR_HIP(hipMalloc((void**)&a->rpart, (size_t)nb * 8 * NP * sizeof(double)));
     const unsigned g1 = (unsigned)((nb + 255) / 256);
-    const unsigned gq = (unsigned)((nb * (NP + 1) + 255) / 256);
+    // workgroup sizes of the direction sweep and the balance-row kernel
+    // (launch knobs; the per-lane arithmetic does not depend on them)
+    const int wq = a->ctx->red_dprops_wg, wo = a->ctx->red_own_wg;
+    const unsigned gq = (unsigned)((nb * (NP + 1) + wq - 1) / wq);
     // the one-wave-per-SIMD kernels run in one-wave workgroups, so each SIMD
     // takes its next wave as soon as its own finishes
     if (fwd_local) {
@@ -2508,9 +2511,9 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
         hipLaunchKernelGGL(osr::k_red_locel<NC>, dim3((unsigned)((nb * osr::RED_GROUP + 255) / 256)), dim3(256), 0,
                            st, A, F, S, t);
     }
-    hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(256), 0, st, A, (const double*)S, (const double*)t,
+    hipLaunchKernelGGL(osr::k_red_dprops<NC>, dim3(gq), dim3(wq), 0, st, A, (const double*)S, (const double*)t,
                        a->dred);
-    hipLaunchKernelGGL(osr::k_red_own<NC>, dim3((unsigned)((nb * (NP + 1) + 63) / 64)), dim3(64), 0, st, A,
+    hipLaunchKernelGGL(osr::k_red_own<NC>, dim3((unsigned)((nb * (NP + 1) + wo - 1) / wo)), dim3(wo), 0, st, A,
                        (const double*)a->dred, browptr, bcols,
                        bvals, a->rpart);
     hipLaunchKernelGGL(osr::k_red_rhs<NC>, dim3(g1), dim3(256), 0, st, A, F, (const double*)a->rpart, brhs);

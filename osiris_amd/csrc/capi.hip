@@ -146,6 +146,9 @@ int osr_tune_set(osr_ctx* ctx, const char* key, double value) {
     } else if (k == "flash.park_after") {
         if (value < 0 || value > 500) return OSR_E_ARG;
         ctx->park_after = (int)value;
+    } else if (k == "reduced.dprops_wg" || k == "reduced.own_wg") {
+        if (value != 64 && value != 128 && value != 256) return OSR_E_ARG;
+        (k == "reduced.dprops_wg" ? ctx->red_dprops_wg : ctx->red_own_wg) = (int)value;
     } else if (k == "assembly.xcd_remap") {
         if (value != 0 && value != 1) return OSR_E_ARG;
         ctx->xcd_remap = (int)value;

@@ -46,6 +46,12 @@ struct osr_ctx {
     // reduced-system kernels: 1 = each XCD takes a contiguous eighth of the
     // grid's workgroups (neighbouring cells share an L2), 0 = dispatch order
     int xcd_remap = 0;
+    // workgroup sizes (64, 128 or 256 lanes) of the reduced system's
+    // direction sweep and balance-row kernels; the one-wave-per-SIMD
+    // balance-row kernel defaults to one-wave workgroups, so each SIMD takes
+    // its next wave as soon as its own finishes
+    int red_dprops_wg = 256;
+    int red_own_wg = 64;
     int mixed_warm_iters = 3;
     float mixed_tol32 = 1e-4f;
     int mixed_max32 = 150;

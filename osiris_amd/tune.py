@@ -13,11 +13,22 @@ reference; they change scheduling only, never the arithmetic of a cell):
   flash.mixed_tol32       FP32 stage stop of the mixed path
   flash.mixed_max32       FP32 stage pass cap of the mixed path
 
+and for the reduced (nc+1)^2 system assembled straight from the state:
+
+  reduced.local_rows      1: local rows from closed-form EOS derivatives,
+                          eliminated in one lane group per block; 0: forward
+                          mode per (block, unknown), then the elimination
+  reduced.dprops_wg       workgroup size of the direction sweep (64/128/256)
+  reduced.own_wg          workgroup size of the balance-row kernel
+  assembly.xcd_remap      1: each XCD walks a contiguous eighth of the grid
+
 `tune_flash` times each candidate on the caller's own cells with HIP events
 (median of a few launches) and walks the knobs one at a time, keeping the
-best value of each (coordinate descent from the built-in defaults).  The
-result is a plain dict that `apply` sets on any context of the same fluid
-and `save`/`load` keep as JSON.
+best value of each (coordinate descent from the built-in defaults);
+`tune_assembly` does the same with the reduced-system call of an assembler
+on the caller's state.  The result is a plain dict that `apply` sets on any
+context of the same fluid and `save`/`load` keep as JSON.  Apply it before
+a graph is captured: a captured launch keeps the shape it was recorded with.
 """
 from __future__ import annotations
 
@@ -32,7 +43,14 @@ DEFAULTS = {
               "flash.mixed_max32": 150},
 }
 
+ASM_DEFAULTS = {"reduced.local_rows": 1, "reduced.dprops_wg": 256, "reduced.own_wg": 64,
+                "assembly.xcd_remap": 0}
+
 SPACE = {
+    "reduced.local_rows": [0, 1],
+    "reduced.dprops_wg": [64, 128, 256],
+    "reduced.own_wg": [64, 128, 256],
+    "assembly.xcd_remap": [0, 1],
     "flash.refill_min": [8, 16, 32, 48, 64],
     "flash.park_after": [0, 16, 24, 40, 64, 100, 160, 250],
     "flash.blocks_per_cu": [0, 1, 2, 3, 4, 6],
@@ -87,6 +105,55 @@ def tune_flash(ctx, T, P, z, mode="persistent", reps=5, sweeps=1, log=None):
                     cfg[key] = v
             ctx.tune(key, cfg[key])
     return {"mode": mode, "config": cfg, "ms": best, "trials": trials}
+
+
+def _descend(ctx, cfg, timer, sweeps, log):
+    for k, v in cfg.items():
+        ctx.tune(k, v)
+    best = timer()
+    trials = [(dict(cfg), best)]
+    for _ in range(sweeps):
+        for key in cfg:
+            for v in SPACE[key]:
+                if v == cfg[key]:
+                    continue
+                ctx.tune(key, v)
+                ms = timer()
+                trials.append(({**cfg, key: v}, ms))
+                if log:
+                    log(f"{key}={v}: {ms:.3f} ms (best {best:.3f})")
+                if ms < best:
+                    best = ms
+                    cfg[key] = v
+            ctx.tune(key, cfg[key])
+    return best, trials
+
+
+def tune_assembly(ctx, asm, d, F, reps=3, sweeps=1, log=None):
+    """Coordinate descent over the reduced system's launch knobs, timing the
+    call asm.reduced_direct(d, F) (HIP events, median of `reps`) on the
+    caller's state.  Returns {"mode": "reduced", "config", "ms", "trials"}."""
+    import torch
+    out = asm.reduced_direct(d, F)
+
+    def timer():
+        stream = torch.cuda.current_stream()
+        asm.reduced_direct(d, F, out=out)
+        times = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            asm.reduced_direct(d, F, out=out)
+            e1.record(stream)
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1))
+        times.sort()
+        return times[len(times) // 2]
+
+    cfg = dict(ASM_DEFAULTS)
+    best, trials = _descend(ctx, cfg, timer, sweeps, log)
+    return {"mode": "reduced", "config": cfg, "ms": best, "trials": trials}
 
 
 def apply(ctx, result):

@@ -257,3 +257,41 @@ def test_direct_reduction_on_the_differentiation_grid(gpu, oracle):
     # direct form differentiates along the eliminated directions)
     assert res["direct"].max() <= 1e-8, res["direct"].max()
     assert res["csr"].max() <= 1e-7, res["csr"].max()
+
+
+def test_reduced_launch_knobs(gpu, oracle):
+    """The reduced system's launch knobs change scheduling, not arithmetic:
+    under every workgroup size of the direction sweep and the balance-row
+    kernel and either workgroup order, the blocks, right-hand sides and S, t
+    are bitwise those of the defaults; the forward-mode local rows
+    (reduced.local_rows = 0) agree with the closed-form ones to rounding
+    (1e-9 of the block row's largest magnitude; S and t to 1e-9 of their
+    largest magnitude)."""
+    import torch
+    fl = synth.fluid_pseudo(6, synth.PR, 3)
+    grid, st = synth.dual_porosity_case(fl, 12, 10, 4, 9, oracle_flash_fn(oracle, fl), p_range=(8e6, 20e6))
+    ctx = gpu.Context(fl)
+    asm = gpu.Assembler(ctx, fl, grid)
+    d = asm.device_state(st)
+    asm.layout(d)
+    F = asm.residual(d)
+    base = [x.clone() for x in asm.reduced_direct(d, F)]
+    for key, vals in (("reduced.dprops_wg", (64, 128)), ("reduced.own_wg", (128, 256)),
+                      ("assembly.xcd_remap", (1,))):
+        for v in vals:
+            ctx.tune(key, v)
+            got = asm.reduced_direct(d, F)
+            for a, b in zip(base, got):
+                assert torch.equal(a.nan_to_num(7.0), b.nan_to_num(7.0)), (key, v)
+        ctx.tune(key, {"reduced.dprops_wg": 256, "reduced.own_wg": 64, "assembly.xcd_remap": 0}[key])
+    ctx.tune("reduced.local_rows", 0)
+    fwd = asm.reduced_direct(d, F)
+    ctx.tune("reduced.local_rows", 1)
+    np1 = fl.nc + 1
+    bv0, bv1 = base[2].reshape(-1, np1, np1), fwd[2].reshape(-1, np1, np1)
+    e = ((bv0 - bv1).abs().amax(dim=2) / bv0.abs().amax(dim=2).clamp_min(1e-300)).max()
+    assert float(e) <= 1e-9, float(e)
+    for a, b in ((base[3], fwd[3]), (base[4], fwd[4]), (base[5], fwd[5])):
+        assert float((a - b).abs().max() / a.abs().max()) <= 1e-9
+    asm.close()
+    ctx.close()
