@@ -234,6 +234,47 @@ int osr_asm_set_workspace(osr_asm *a, void *ptr, size_t bytes);
  * the reference's sweeps fill (blockFProps and kin, Do_Cycle.cpp:99-437). */
 int osr_asm_props_table(osr_asm *a, const double **props);
 
+/* Wells (deck WELLS, Data_Input.cpp:1196-1330, with the run's simTime and
+ * WellCondition).  welli rows as the reference stores them: type 0..8, then
+ * the 0-based block Ix, Iy and the first and last completed layer; wellf rows
+ * of nc + 6 values in the type's order (Pwf or rate, limit, skin, rw, for
+ * types 3 and 8 the nc injection mole fractions, then start and end time,
+ * s).  Host memory. */
+typedef struct osr_wells {
+    int32_t nw;
+    const int32_t *welli;    /* [nw*5] */
+    const double *wellf;     /* [nw*(nc+6)] */
+    const int8_t *condition; /* [nw] WellCondition, NULL = all false */
+    double sim_time;         /* simTime, s */
+    double re_f;             /* Re_f, m */
+} osr_wells;
+
+/* Completes the wells in the assembler's grid: from then on the residual,
+ * Jacobian and reduced-system calls add the well terms of IteratorFunction
+ * (Petsc_Functions.cpp:164-697 fracture blocks, :3505-4040 matrix blocks;
+ * the rates Qw, Qo, Qg of the last active well of a block enter its
+ * component rows as x_i Qo + y_i Qg -- an injector's stream compositions
+ * instead, :787-808 --, its total row as Qo + Qg and its water row as Qw).
+ * Computes each well's equivalent radius (Calc_Re, Minor_NR.cpp:2700-3249;
+ * the matrix blocks use it, the fracture blocks Re_f) on the host and the
+ * injection stream's flash (FlashInjection, Init_Calcs.cpp:1161-1377) of
+ * types 3 and 8 on the device, on `stream`; returns when both are done.
+ * The allocation sums over a well's layers (SSo1 ...) read the other
+ * blocks' property-table values, so the Jacobian takes the derivative with
+ * respect to the block's own unknowns, as the reference's does.  Call again
+ * whenever simTime or WellCondition change (a call with the same number of
+ * wells and completed blocks keeps the device tables, so a graph captured
+ * earlier reads the new values); nw = 0 removes the wells.  The reference-
+ * mode Jacobian (osr_jacobian_ref_*) refuses an assembler with wells
+ * (OSR_E_ARG). */
+int osr_asm_set_wells(osr_asm *a, const osr_wells *w, void *stream);
+
+/* Per-well results of the last osr_asm_set_wells (host outputs, either may
+ * be NULL): re[nw] the equivalent radius; inj[nw*(2nc+6)] the injection
+ * stream's x (nc), y (nc), oil and gas fraction, densities (mol/m^3) and
+ * viscosities -- the outputs of FlashInjection; zeros for the other types. */
+int osr_asm_well_data(osr_asm *a, double *re, double *inj);
+
 /* Block offsets of the layout for the state's labels (PJac).  offsets
  * (device, may be NULL) receives 2n+1 entries; *nrows (host) = MatrixSize.
  * Must be called after any label change, before residual/Jacobian. */

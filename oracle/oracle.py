@@ -158,6 +158,15 @@ class _FluidExt(C.Structure):
     _fields_ = [("MW", _dp), ("Vc", _dp), ("Zc", _dp), ("parachor", _dp)]
 
 
+_i8p = C.POINTER(C.c_int8)
+_i32p = C.POINTER(C.c_int32)
+
+
+class _Wells(C.Structure):
+    _fields_ = [("nw", C.c_int32), ("welli", _i32p), ("wellf", _dp), ("condition", _i8p),
+                ("sim_time", C.c_double), ("re_f", C.c_double)]
+
+
 class _Grid(C.Structure):
     _fields_ = [("dx", _dp), ("dy", _dp), ("dz", _dp), ("phi_m", _dp), ("phi_f", _dp),
                 ("perm_m", _dp), ("perm_f", _dp), ("tor", _dp), ("swt", _dp), ("sgt", _dp),
@@ -165,10 +174,8 @@ class _Grid(C.Structure):
                 ("sigma", C.c_double), ("lf", C.c_double), ("cdiff", C.c_double),
                 ("T", C.c_double), ("dt", C.c_double), ("wat_ro", C.c_double),
                 ("wat_mu", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
-                ("const_diff", C.c_int32), ("nswt", C.c_int32), ("nsgt", C.c_int32)]
-
-
-_i8p = C.POINTER(C.c_int8)
+                ("const_diff", C.c_int32), ("nswt", C.c_int32), ("nsgt", C.c_int32),
+                ("wells", C.POINTER(_Wells))]
 
 
 class _State(C.Structure):
@@ -191,13 +198,18 @@ class _Asm:
                 return None
             a = np.ascontiguousarray(a, dtype=dt)
             self.keep.append(a)
-            return a.ctypes.data_as(_dp if dt == np.float64 else _i8p)
+            return a.ctypes.data_as({np.float64: _dp, np.int8: _i8p, np.int32: _i32p}[dt])
 
         self.ext = _FluidExt(P(fluid.MW), P(fluid.Vc), P(fluid.Zc), P(fluid.parachor))
         self.grid = _Grid(P(g.dx), P(g.dy), P(g.dz), P(g.phi_m), P(g.phi_f), P(g.perm_m),
                           P(g.perm_f), P(g.tor), P(g.swt), P(g.sgt), g.refP, g.cpor, g.dcpor,
                           g.sigma, g.lf, g.cdiff, g.T, g.dt, g.wat_ro, g.wat_mu, g.nx, g.ny,
-                          g.nz, int(g.const_diff), len(g.swt), len(g.sgt))
+                          g.nz, int(g.const_diff), len(g.swt), len(g.sgt), None)
+        w = getattr(g, "wells", None)
+        if w is not None and w.nw > 0:
+            self.wells = _Wells(w.nw, P(w.welli, np.int32), P(w.wellf), P(w.condition, np.int8),
+                                float(w.sim_time), float(w.re_f))
+            self.grid.wells = C.pointer(self.wells)
         s = state
         self.state = _State(P(s.label_m, np.int8), P(s.label_f, np.int8), P(s.Po_m), P(s.Po_f),
                             P(s.x_m), P(s.y_m), P(s.x_f), P(s.y_f), P(s.S_m), P(s.S_f),
@@ -230,6 +242,17 @@ def props(fluid, grid, state):
     out = np.zeros((2, a.n, w))
     lib().or_props(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), C.byref(a.state), _p(out))
     return out
+
+
+def well_data(fluid, grid, state):
+    """The grid's wells: (re [nw] from Calc_Re, inj [nw, 2nc + 6] from
+    FlashInjection: x, y, oil/gas fraction, density, viscosity)."""
+    a = _Asm(fluid, grid, state)
+    nw = grid.wells.nw
+    re = np.zeros(nw)
+    inj = np.zeros((nw, 2 * a.nc + 6))
+    lib().or_well_data(a.fluid.ref, C.byref(a.ext), C.byref(a.grid), _p(re), _p(inj))
+    return re, inj
 
 
 def iftran(fluid, grid, state):

@@ -67,6 +67,16 @@ int or_residual(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, con
     return 0;
 }
 
+// Per-well Calc_Re and FlashInjection outputs of the grid's wells:
+// re[nw], inj[nw][2nc + 6] (zeros for wells that are not types 3 and 8).
+int or_well_data(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, double* re, double* inj) {
+    Model M;
+    build_model(M, f, fe, g);
+    for (int i = 0; i < M.nw; i++) re[i] = M.well_re[i];
+    for (size_t i = 0; i < M.inj.size(); i++) inj[i] = M.inj[i];
+    return M.nw;
+}
+
 // Per-row sum of the magnitudes of the terms the residual row adds up.
 int or_residual_scale(const or_fluid* f, const or_fluid_ext* fe, const or_grid* g, const or_state* s, double* F) {
     g_absmode = true;

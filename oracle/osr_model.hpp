@@ -49,7 +49,18 @@ struct Model {
     std::vector<double> phi[2], perm[2], tor;
     std::vector<double> iftran[2][3];  // kA/h of the face to the +dir neighbour
     double d1, d2;
+    std::vector<double> om;             // acentric factors (FlashInjection's Wilson start)
+    // wells (deck WELLS and the run's simTime / WellCondition)
+    int nw = 0;
+    std::vector<int> welli;             // [nw][5] type, Ix, Iy, first and last layer
+    std::vector<double> wellf;          // [nw][nc + 6]
+    std::vector<int8_t> wcond;          // [nw] WellCondition
+    double sim_time = 0, re_f = 0;
+    std::vector<double> well_re;        // [nw] Calc_Re of the well's block (matrix completions)
+    std::vector<double> inj;            // [nw][2nc + 6] FlashInjection of types 3 and 8
 };
+
+void well_setup(Model& M, const or_wells* w);
 
 inline int cid(const Model& M, int ix, int iy, int iz) { return iz * M.nx * M.ny + iy * M.nx + ix; }
 
@@ -97,6 +108,7 @@ void build_model(Model& M, const or_fluid* f, const or_fluid_ext* fe, const or_g
     }
     if (M.eos == OR_SRK) { M.d1 = 1; M.d2 = 0; }
     else { M.d1 = 1 + OR_SQRT2; M.d2 = 1 - OR_SQRT2; }
+    M.om.assign(f->w, f->w + nc);
     const int n = M.n;
     M.phi[0].assign(g->phi_m, g->phi_m + n);
     M.phi[1].assign(g->phi_f, g->phi_f + n);
@@ -127,6 +139,7 @@ void build_model(Model& M, const or_fluid* f, const or_fluid_ext* fe, const or_g
                                             (M.dz[iz] / pm[2 * n + c] + M.dz[iz + 1] / pm[2 * n + cz]);
                 }
             }
+    if (g->wells && g->wells->nw > 0) well_setup(M, g->wells);
 }
 
 // ------------------------------------------------------------ cell state
@@ -437,6 +450,570 @@ Face<T> face_terms(const Model& M, int k, int d, int ca, int cb, double da, doub
     return f;
 }
 
+// ---------------------------------------------------------------- wells
+// Wells of IteratorFunction (Petsc_Functions.cpp:164-697 fracture blocks,
+// :3505-4040 matrix blocks; the liquid and gas blocks repeat the same code at
+// :1329, :2421, :4683, :5810), Calc_Re (Minor_NR.cpp:2700-3249) and
+// FlashInjection (Init_Calcs.cpp:1161-1377).
+constexpr double PI_R = 3.1415926535897932384626433;  // Functions.h:218
+constexpr double WELL_MIN_P = 7000000;                // Functions.h:6
+constexpr double WELL_MAX_P = 14000000;               // Functions.h:5
+constexpr double TSTEPIF = 1e-5;                      // Functions.h:265
+constexpr int W_OF = 0, W_GF = 1, W_OD = 2, W_GD = 3, W_OV = 4, W_GV = 5;  // after the 2 nc compositions
+
+// Calc_Re (Minor_NR.cpp:2700-3249): equivalent radius of the well in block
+// (Ix, Iy) from the interface transmissibility factors and the image wells
+// of the reservoir boundaries.  The well sits at X_posi = Y_posi = 0 of its
+// block; the branches run in the reference's order on shared factors (on a
+// one-block-wide grid more than one branch runs, each on what the previous
+// left, as there).
+double calc_re(const Model& M, int Ix, int Iy) {
+    const double dxi = M.dx[Ix], dyi = M.dy[Iy];
+    const int Nx = M.nx, Ny = M.ny;
+    const double X = 0, Y = 0;
+    double T[9], r1[9], rij[5][9], aj[5];
+    for (int j = 0; j < 9; j++) { T[j] = 0; r1[j] = 0; for (int a = 0; a < 5; a++) rij[a][j] = 0; }
+    for (int a = 0; a < 5; a++) aj[a] = 0;
+    double bSum = 0, iZ = 1, jZ = 1, Re = 0, frac = 1;
+    using std::sqrt; using std::pow; using std::exp;
+    r1[1] = sqrt(X * X + (dyi + Y) * (dyi + Y));
+    r1[2] = sqrt((dxi - X) * (dxi - X) + Y * Y);
+    r1[3] = sqrt(X * X + (dyi - Y) * (dyi - Y));
+    r1[4] = sqrt((dxi + X) * (dxi + X) + Y * Y);
+    r1[5] = sqrt((dxi + X) * (dxi + X) + (dyi + Y) * (dyi + Y));
+    r1[6] = sqrt((dxi - X) * (dxi - X) + (dyi + Y) * (dyi + Y));
+    r1[7] = sqrt((dxi - X) * (dxi - X) + (dyi - Y) * (dyi - Y));
+    r1[8] = sqrt((dxi + X) * (dxi + X) + (dyi - Y) * (dyi - Y));
+    T[5] = T[6] = T[7] = T[8] = 0;
+    T[1] = dxi / dyi - (T[5] + T[6]);
+    T[2] = dyi / dxi - (T[6] + T[7]);
+    T[3] = dxi / dyi - (T[7] + T[8]);
+    T[4] = dyi / dxi - (T[5] + T[8]);
+    auto sum_b = [&]() { for (int j = 1; j < 9; j++) bSum += T[j]; bSum = 1 / bSum; };
+    auto edge = [&](const int* zero) {
+        // one image well: rij[2][.] / aj[2]
+        for (int q = 0; q < 3; q++) { rij[2][zero[q]] = 1; r1[zero[q]] = 1; T[zero[q]] = 0; }
+        sum_b();
+        for (int j = 1; j < 5; j++) iZ *= pow(r1[j], T[j]) * pow(rij[2][j] / aj[2], T[j]);
+        Re = pow(exp(-2 * PI_R * frac) * iZ, bSum);
+    };
+    auto corner = [&](const int* keep) {
+        // three image wells: rij[2..4][.] / aj[2..4]
+        for (int j = 1; j < 9; j++)
+            if (j != keep[0] && j != keep[1] && j != keep[2]) {
+                T[j] = 0; r1[j] = 1; rij[2][j] = 1; rij[3][j] = 1; rij[4][j] = 1;
+            }
+        sum_b();
+        iZ = 1;
+        for (int t = 1; t < 5; t++) {
+            jZ = 1;
+            for (int j = 2; j < 5; j++) jZ *= pow(rij[j][t] / aj[j], T[t]);
+            iZ *= pow(r1[t], T[t]) * jZ;
+        }
+        Re = pow(exp(-2 * PI_R * frac) * iZ, bSum);
+    };
+    if (0 < Ix && Ix < Nx - 1 && 0 < Iy && Iy < Ny - 1) {   // interior block
+        frac = 1;
+        for (int j = 1; j < 5; j++) iZ *= pow(r1[j], T[j]);
+        sum_b();
+        Re = pow(exp(-2 * PI_R * frac) * iZ, bSum);
+    }
+    if (Iy == Ny - 1 && Ix != 0 && Ix != Nx - 1) {          // boundary Iy = Ny - 1
+        frac = 1;
+        aj[2] = dyi + 2 * Y;
+        rij[2][2] = sqrt((dxi - X) * (dxi - X) + (dyi + Y) * (dyi + Y));
+        rij[2][7] = sqrt((dxi - X) * (dxi - X) + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[2][3] = sqrt(X * X + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[2][8] = sqrt((dxi + X) * (dxi + X) + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[2][4] = sqrt((dxi + X) * (dxi + X) + (dyi + Y) * (dyi + Y));
+        const int z[3] = {1, 5, 6};
+        edge(z);
+    }
+    if (Iy == 0 && Ix != 0 && Ix != Nx - 1) {               // boundary Iy = 0
+        frac = 1;
+        aj[2] = dyi - 2 * Y;
+        rij[2][2] = sqrt((dxi - X) * (dxi - X) + (dyi - Y) * (dyi - Y));
+        rij[2][6] = sqrt((dxi - X) * (dxi - X) + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[2][1] = sqrt(X * X + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[2][5] = sqrt((dxi + X) * (dxi + X) + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[2][4] = sqrt((dxi + X) * (dxi + X) + (dyi - Y) * (dyi - Y));
+        const int z[3] = {3, 7, 8};
+        edge(z);
+    }
+    if (Ix == Nx - 1 && Iy != 0 && Iy != Ny - 1) {          // boundary Ix = Nx - 1
+        frac = 1;
+        aj[2] = dxi - 2 * X;
+        rij[2][1] = sqrt((dxi - X) * (dxi - X) + (dyi + Y) * (dyi + Y));
+        rij[2][5] = sqrt((2 * dxi - X) * (2 * dxi - X) + (dyi + Y) * (dyi + Y));
+        rij[2][4] = sqrt((2 * dxi - X) * (2 * dxi - X) + Y * Y);
+        rij[2][8] = sqrt((2 * dxi - X) * (2 * dxi - X) + (dyi - Y) * (dyi - Y));
+        rij[2][3] = sqrt((dxi - X) * (dxi - X) + (dyi - Y) * (dyi - Y));
+        const int z[3] = {2, 7, 6};
+        edge(z);
+    }
+    if (Ix == 0 && Iy != 0 && Iy != Ny - 1) {               // boundary Ix = 0
+        frac = 1;
+        aj[2] = dxi + 2 * X;
+        rij[2][1] = sqrt((dxi + X) * (dxi + X) + (dyi + Y) * (dyi + Y));
+        rij[2][7] = sqrt((2 * dxi + X) * (2 * dxi + X) + (dyi - Y) * (dyi - Y));
+        rij[2][2] = sqrt((2 * dxi + X) * (2 * dxi + X) + Y * Y);
+        rij[2][6] = sqrt((2 * dxi + X) * (2 * dxi + X) + (dyi + Y) * (dyi + Y));
+        rij[2][3] = sqrt((dxi + X) * (dxi + X) + (dyi - Y) * (dyi - Y));
+        const int z[3] = {8, 4, 5};
+        edge(z);
+    }
+    if (Ix == 0 && Iy == Ny - 1) {                          // corner
+        frac = 1;
+        aj[2] = dyi + 2 * Y;
+        aj[4] = dxi + 2 * X;
+        aj[3] = sqrt((dyi + 2 * Y) * (dyi + 2 * Y) + (dxi + 2 * X) * (dxi + 2 * X));
+        rij[2][2] = sqrt((dxi - X) * (dxi - X) + (dyi + Y) * (dyi + Y));
+        rij[2][7] = sqrt((dxi - X) * (dxi - X) + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[2][3] = sqrt(X * X + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[3][2] = sqrt((2 * dxi + X) * (2 * dxi + X) + (dyi + Y) * (dyi + Y));
+        rij[3][3] = sqrt((dxi + X) * (dxi + X) + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[3][7] = sqrt((2 * dxi + X) * (2 * dxi + X) + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[4][2] = sqrt((2 * dxi + X) * (2 * dxi + X) + Y * Y);
+        rij[4][3] = sqrt((dxi + X) * (dxi + X) + (dyi - Y) * (dyi - Y));
+        rij[4][7] = sqrt((2 * dxi + X) * (2 * dxi + X) + (dyi - Y) * (dyi - Y));
+        const int k3[3] = {2, 3, 7};
+        corner(k3);
+    }
+    if (Ix == 0 && Iy == 0) {                               // corner
+        frac = 1;
+        aj[2] = dyi - 2 * Y;
+        aj[4] = dxi + 2 * X;
+        aj[3] = sqrt((dyi - 2 * Y) * (dyi - 2 * Y) + (dxi + 2 * X) * (dxi + 2 * X));
+        rij[2][2] = sqrt((dxi - X) * (dxi - X) + (dyi - Y) * (dyi - Y));
+        rij[2][6] = sqrt((dxi - X) * (dxi - X) + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[2][1] = sqrt(X * X + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[3][2] = sqrt((2 * dxi + X) * (2 * dxi + X) + (dyi - Y) * (dyi - Y));
+        rij[3][1] = sqrt((dxi + X) * (dxi + X) + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[3][6] = sqrt((2 * dxi + X) * (2 * dxi + X) + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[4][2] = sqrt((2 * dxi + X) * (2 * dxi + X) + Y * Y);
+        rij[4][1] = sqrt((dxi + X) * (dxi + X) + (dyi + Y) * (dyi + Y));
+        rij[4][6] = sqrt((2 * dxi + X) * (2 * dxi + X) + (dyi + Y) * (dyi + Y));
+        const int k3[3] = {1, 2, 6};
+        corner(k3);
+    }
+    if (Ix == Nx - 1 && Iy == Ny - 1) {                     // corner
+        frac = 1;
+        aj[2] = dxi - 2 * X;
+        aj[4] = dyi + 2 * Y;
+        aj[3] = sqrt((dxi - 2 * X) * (dxi - 2 * X) + (dyi + 2 * Y) * (dyi + 2 * Y));
+        rij[2][4] = sqrt((2 * dxi - X) * (2 * dxi - X) + Y * Y);
+        rij[2][8] = sqrt((2 * dxi - X) * (2 * dxi - X) + (dyi - Y) * (dyi - Y));
+        rij[2][3] = sqrt((dxi - X) * (dxi - X) + (dyi - Y) * (dyi - Y));
+        rij[3][4] = sqrt((2 * dxi - X) * (2 * dxi - X) + (dyi + Y) * (dyi + Y));
+        rij[3][3] = sqrt((dxi - X) * (dxi - X) + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[3][8] = sqrt((2 * dxi - X) * (2 * dxi - X) + (2 * dyi + Y) * (2 * dyi + Y));
+        rij[4][3] = sqrt((2 * dyi + Y) * (2 * dyi + Y) + X * X);
+        rij[4][4] = sqrt((dxi + X) * (dxi + X) + (dyi + Y) * (dyi + Y));
+        rij[4][8] = sqrt((dxi + X) * (dxi + X) + (2 * dyi + Y) * (2 * dyi + Y));
+        const int k3[3] = {3, 4, 8};
+        corner(k3);
+    }
+    if (Ix == Nx - 1 && Iy == 0) {                          // corner
+        frac = 1;
+        aj[2] = dxi - 2 * X;
+        aj[4] = dyi - 2 * Y;
+        aj[3] = sqrt((dxi - 2 * X) * (dxi - 2 * X) + (dyi - 2 * Y) * (dyi - 2 * Y));
+        rij[2][1] = sqrt((dxi - X) * (dxi - X) + (dyi + Y) * (dyi + Y));
+        rij[2][5] = sqrt((2 * dxi - X) * (2 * dxi - X) + (dyi + Y) * (dyi + Y));
+        rij[2][4] = sqrt((2 * dxi - X) * (2 * dxi - X) + Y * Y);
+        rij[3][4] = sqrt((2 * dxi - X) * (2 * dxi - X) + (dyi - Y) * (dyi - Y));
+        rij[3][1] = sqrt((dxi - X) * (dxi - X) + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[3][5] = sqrt((2 * dxi - X) * (2 * dxi - X) + (2 * dyi - Y) * (2 * dyi - Y));
+        rij[4][1] = sqrt((2 * dyi - Y) * (2 * dyi - Y) + X * X);
+        rij[4][4] = sqrt((dxi + X) * (dxi + X) + (dyi - Y) * (dyi - Y));
+        rij[4][5] = sqrt((dxi + X) * (dxi + X) + (2 * dyi - Y) * (2 * dyi - Y));
+        const int k3[3] = {1, 4, 5};
+        corner(k3);
+    }
+    return Re;
+}
+
+// FlashInjection (Init_Calcs.cpp:1161-1377): successive substitution on the
+// injection stream z at pressure P from Wilson K-values, the Rachford-Rice
+// Newton restarted at L = 0.6 every pass, until L moves by at most
+// FLASH_TOL; then the phase split, densities and LBC viscosities.  A K-value
+// turned negative makes the vapour composition y_i / (1 - L) of the previous
+// pass's y_i (the reference divides the output array, :1219).
+// out: [x (nc), y (nc), OilFraction, GasFraction, OilDensity, GasDensity,
+//       OilViscosity, GasViscosity]
+void flash_injection(const Model& M, const double* z, double P, double* out) {
+    const int nc = M.nc;
+    const double RT = RG * M.T;
+    double K[NCM], x[NCM], y[NCM];
+    for (int i = 0; i < nc; i++) {
+        K[i] = (M.pc[i] / P) * std::exp(5.37 * (1 + M.om[i]) * (1 - M.tc[i] / M.T));
+        x[i] = 0;
+        y[i] = 0;
+    }
+    double L = 0.6, Lb = 10, ESum, Zl = 0, Zg = 0;
+    int k = 0;
+    const bool srk = M.eos == OR_SRK;
+    do {
+        L = 0.6;
+        int j = 0;
+        double tD;
+        do {
+            double F = 0, dF = 0;
+            for (int i = 0; i < nc; i++) {
+                if (K[i] >= 0) tD = (1 - K[i]) / (L + (1 - L) * K[i]);
+                else tD = 1 / (L - 1);
+                F += z[i] * tD;
+                dF -= z[i] * tD * tD;
+            }
+            tD = F / dF;
+            L -= tD;
+            j++;
+        } while ((tD * tD) > 1e-8 && j < 100);   // NEWTON_TOL, MAX_FLASH_N_ITR
+        if (L > 1) L = 1;
+        else if (L < 0) L = 0;
+        ESum = std::fabs(Lb - L);
+        Lb = L;
+        for (int i = 0; i < nc; i++) {
+            if (K[i] >= 0) {
+                x[i] = z[i] / (L + K[i] * (1 - L));
+                y[i] = K[i] * x[i];
+            } else {
+                x[i] = 0;
+                y[i] = y[i] / (1 - L);
+            }
+        }
+        double Al = 0, Bl = 0, Ag = 0, Bg = 0;
+        for (int i = 0; i < nc; i++) {
+            Bl += x[i] * M.b[i];
+            Bg += y[i] * M.b[i];
+            for (int jj = 0; jj < nc; jj++) {
+                const double sq = std::sqrt(M.a[i] * M.a[jj]) * M.bic[i * nc + jj];
+                Al += x[i] * x[jj] * sq;
+                Ag += y[i] * y[jj] * sq;
+            }
+        }
+        const double Cal = Al, Cbl = Bl, Cag = Ag, Cbg = Bg;
+        Al *= P / (RG * RG * M.T * M.T);
+        Bl *= P / RT;
+        Ag *= P / (RG * RG * M.T * M.T);
+        Bg *= P / RT;
+        const double Ul = srk ? Bl : 2 * Bl, Wl = srk ? 0 : Bl;
+        const double Ug = srk ? Bg : 2 * Bg, Wg = srk ? 0 : Bg;
+        Zl = or_solve_z(-(1 + Bl - Ul), Al - Bl * Ul - Ul - Wl * Wl, -(Al * Bl - Bl * Wl * Wl - Wl * Wl), 0);
+        Zg = or_solve_z(-(1 + Bg - Ug), Ag - Bg * Ug - Ug - Wg * Wg, -(Ag * Bg - Bg * Wg * Wg - Wg * Wg), 1);
+        for (int i = 0; i < nc; i++) {
+            double tl = 0, tg = 0;
+            for (int jj = 0; jj < nc; jj++) {
+                const double sq = std::sqrt(M.a[i] * M.a[jj]) * M.bic[i * nc + jj];
+                tl += x[jj] * sq;
+                tg += y[jj] * sq;
+            }
+            const double t1l = M.b[i] / Cbl, t1g = M.b[i] / Cbg;
+            const double EEl = 2 * tl / Cal - t1l, EEg = 2 * tg / Cag - t1g;
+            const double DDl = Al / (Bl * (M.d1 - M.d2)), DDg = Ag / (Bg * (M.d1 - M.d2));
+            const double FFl = std::log((Zl + M.d2 * Bl) / (Zl + M.d1 * Bl));
+            const double FFg = std::log((Zg + M.d2 * Bg) / (Zg + M.d1 * Bg));
+            const double phiL = std::exp(t1l * (Zl - 1) - std::log(Zl - Bl) + DDl * EEl * FFl);
+            const double phiG = std::exp(t1g * (Zg - 1) - std::log(Zg - Bg) + DDg * EEg * FFg);
+            const double fL = x[i] * phiL, fG = y[i] * phiG;
+            if (z[i] != 0) {
+                if (fG != 0) K[i] *= fL / fG;
+                else K[i] = -1;
+                if (fL != fL) K[i] = -1;
+                if (fG != fG) K[i] = 1;
+            }
+        }
+        k++;
+    } while (ESum > 1e-14 && k < 500);   // FLASH_TOL
+    double OF, GF;
+    if (std::fabs(L) < 1e-20) { OF = 0; GF = 1; }
+    else if (std::fabs(1 - L) < 1e-20) { OF = 1; GF = 0; }
+    else { OF = L; GF = 1 - L; }
+    const double OD = P / (Zl * RT), GD = P / (Zg * RT);
+    double mu[2];
+    for (int ph = 0; ph < 2; ph++) {
+        const double* w = ph == 0 ? x : y;
+        const double ro = ph == 0 ? OD : GD;
+        double s1 = 0, s2 = 0, rv = 0, tcm = 0, pcm = 0, mwm = 0;
+        for (int i = 0; i < nc; i++) {
+            const double tm = w[i] * std::sqrt(M.mw[i]);
+            s1 += tm * M.mu0[i];
+            s2 += tm;
+            rv += w[i] * M.vcm3[i];
+            tcm += w[i] * M.tc[i];
+            pcm += w[i] * M.pc[i] / 101325;
+            mwm += w[i] * M.mw[i];
+        }
+        const double Er = ro * rv;
+        const double eta = std::pow(tcm, 1.0 / 6) / (std::sqrt(mwm) * std::pow(pcm, 2.0 / 3));
+        const double r1 = 0.10230 + 0.023364 * Er + 0.058533 * Er * Er - 0.040758 * Er * Er * Er +
+                          0.0093324 * Er * Er * Er * Er;
+        mu[ph] = s1 / s2 + (r1 * r1 * r1 * r1 - 1e-4) / eta;
+    }
+    for (int i = 0; i < nc; i++) { out[i] = x[i]; out[nc + i] = y[i]; }
+    double* t = out + 2 * nc;
+    t[W_OF] = OF; t[W_GF] = GF; t[W_OD] = OD; t[W_GD] = GD; t[W_OV] = mu[0]; t[W_GV] = mu[1];
+}
+
+void well_setup(Model& M, const or_wells* w) {
+    const int nc = M.nc, nw = w->nw;
+    M.nw = nw;
+    M.welli.assign(w->welli, w->welli + 5 * nw);
+    M.wellf.assign(w->wellf, w->wellf + (nc + 6) * nw);
+    M.wcond.assign(nw, 0);
+    if (w->condition) M.wcond.assign(w->condition, w->condition + nw);
+    M.sim_time = w->sim_time;
+    M.re_f = w->re_f;
+    M.well_re.assign(nw, 0);
+    M.inj.assign((size_t)nw * (2 * nc + 6), 0);
+    for (int i = 0; i < nw; i++) {
+        M.well_re[i] = calc_re(M, M.welli[5 * i + 1], M.welli[5 * i + 2]);
+        const int type = M.welli[5 * i];
+        const double* wf = &M.wellf[(size_t)(nc + 6) * i];
+        if (type == 3 || type == 8)
+            flash_injection(M, wf + 4, type == 3 ? wf[0] : wf[1], &M.inj[(size_t)(2 * nc + 6) * i]);
+    }
+}
+
+// Rates of the wells completed in block (k, c) (one evaluation per block, as
+// the reference's loop over wells at :181-697 does it): the last active well
+// that matches sets the rates; an injector (types 3, 8) marks the block so
+// its component rows take the injection stream's compositions.  Rates are
+// positive into the block.  The allocation sums over the well's completions
+// (SSo1 ...) take the other blocks' properties as values: the derivative is
+// the one with respect to the block's own unknowns.
+template <class T>
+struct WellQ {
+    T Qw = T(0), Qo = T(0), Qg = T(0), PIw = T(0), PIo = T(0), PIg = T(0);
+    int inj = -1;   // injector whose stream the component rows take, or -1
+};
+
+template <class T>
+WellQ<T> well_terms(const Model& M, int k, int c, int ix, int iy, int iz, const Props<T>& p,
+                    const std::vector<Props<T>>* Pp) {
+    WellQ<T> q;
+    const int nc = M.nc, n = M.n;
+    const double* pk = M.perm[k].data();
+    const double* px = M.perm[1 - k].data();
+    const Props<T>& pxc = Pp[1 - k][c];
+    auto nan_or = [](const T& v) { const double a = val(v); return (a - a) != 0; };
+    for (int i = 0; i < M.nw; i++) {
+        const int* wi = &M.welli[5 * i];
+        if (!(ix == wi[1] && iy == wi[2] && iz >= wi[3] && iz <= wi[4])) continue;
+        const double* wf = &M.wellf[(size_t)(nc + 6) * i];
+        const int type = wi[0];
+        const int ts = (type == 3 || type == 8) ? nc + 4 : 4;
+        if (!((std::fabs(M.sim_time - wf[ts]) <= TSTEPIF) || (std::fabs(M.sim_time - wf[ts + 1]) <= TSTEPIF) ||
+              ((M.sim_time >= wf[ts]) && (M.sim_time <= wf[ts + 1]))))
+            continue;
+        const double rw = wf[3], skin = wf[2];
+        double Re = k == 1 ? M.re_f : M.well_re[i];
+        if (k == 0 && (type == 0 || type == 3 || type == 8) && Re <= rw) Re += rw;
+        const double Lf = std::log((rw + M.re_f) / rw) + skin;
+        const double Lm = std::log(Re / rw) + skin;
+        const double Lown = k == 1 ? Lf : Lm, Loth = k == 1 ? Lm : Lf;
+        const double S1pi = 2 * SI_C * PI_R * M.dz[iz] * std::sqrt(pk[n + c] * pk[c]) / Lown;
+        const double S1r = M.dz[iz] * std::sqrt(pk[c] * pk[n + c]) / Lown;
+        const bool cond = M.wcond[i] != 0;
+        const T krw = p.kr[0], kro = p.kr[1], krg = p.kr[2];
+        const T roo = p.ro[0], rog = p.ro[1], muo = p.mu[0], mug = p.mu[1];
+        const double watf = M.wat_ro * WAT_M_RO;
+        auto set_pi = [&](double S1) {
+            q.PIw = S1 * M.wat_ro * WAT_M_RO * krw / M.wat_mu;
+            q.PIo = S1 * roo * kro / muo;
+            q.PIg = S1 * rog * krg / mug;
+        };
+        auto pwf_rates = [&](double Pwf) {
+            q.Qw = q.PIw * (Pwf - p.Pw);
+            q.Qo = q.PIo * (Pwf - p.Po);
+            q.Qg = q.PIg * (Pwf - p.Pg);
+        };
+        auto clamp_prod = [&]() {
+            if (val(q.Qw) > 0 || nan_or(q.Qw)) { q.Qw = T(0); q.PIw = T(0); }
+            if (val(q.Qo) > 0 || nan_or(q.Qo)) { q.Qo = T(0); q.PIo = T(0); }
+            if (val(q.Qg) > 0 || nan_or(q.Qg)) { q.Qg = T(0); q.PIg = T(0); }
+        };
+        // completion sums over layers j: own continuum first, then the other;
+        // per completion f(ro, kr, mu of that block) x S2, over mu_den (the
+        // phase viscosity of this layer in the completion's continuum)
+        auto own_at = [&](int j) -> Props<T> {
+            if (j == iz) return p;
+            Props<T> r;
+            const Props<T>& s = Pp[k][cid(M, ix, iy, j)];
+            for (int ph = 0; ph < 2; ph++) { r.ro[ph] = T(val(s.ro[ph])); r.mu[ph] = T(val(s.mu[ph])); }
+            for (int a = 0; a < 3; a++) r.kr[a] = T(val(s.kr[a]));
+            return r;
+        };
+        auto oth_at = [&](int j) -> Props<T> {
+            Props<T> r;
+            const Props<T>& s = Pp[1 - k][cid(M, ix, iy, j)];
+            for (int ph = 0; ph < 2; ph++) { r.ro[ph] = T(val(s.ro[ph])); r.mu[ph] = T(val(s.mu[ph])); }
+            for (int a = 0; a < 3; a++) r.kr[a] = T(val(s.kr[a]));
+            return r;
+        };
+        auto S2own = [&](int j) { const int cj = cid(M, ix, iy, j); return M.dz[j] * std::sqrt(pk[cj] * pk[n + cj]) / Lown; };
+        auto S2oth = [&](int j) { const int cj = cid(M, ix, iy, j); return M.dz[j] * std::sqrt(px[cj] * px[n + cj]) / Loth; };
+        // SSo1 of a phase ph (ro S2 kr / mu of this layer); ph = 2: water (watRo S2 krw / watMu)
+        auto sso1 = [&](int ph) {
+            T s = T(0);
+            const T muown = ph < 2 ? p.mu[ph] : T(0);
+            const T muoth = ph < 2 ? T(val(pxc.mu[ph])) : T(0);
+            for (int j = wi[3]; j <= wi[4]; j++) {
+                const Props<T> a = own_at(j), b = oth_at(j);
+                if (ph < 2) {
+                    s += a.ro[ph] * S2own(j) * a.kr[1 + ph] / muown;
+                    s += b.ro[ph] * S2oth(j) * b.kr[1 + ph] / muoth;
+                } else {
+                    s += M.wat_ro * S2own(j) * a.kr[0] / M.wat_mu;
+                    s += M.wat_ro * S2oth(j) * b.kr[0] / M.wat_mu;
+                }
+            }
+            return s;
+        };
+        auto sso1_geo = [&]() {
+            double s = 0;
+            for (int j = wi[3]; j <= wi[4]; j++) { s += S2own(j); s += S2oth(j); }
+            return s;
+        };
+        const double* ij = &M.inj[(size_t)(2 * nc + 6) * i + 2 * nc];
+        switch (type) {
+            case 0:   // constant Pwf, oil production (:186-266)
+                set_pi(S1pi);
+                pwf_rates(wf[0]);
+                if (val(p.Po) < WELL_MIN_P) {
+                    q.Qw = q.Qo = q.Qg = T(0);
+                    q.PIw = q.PIo = q.PIg = T(0);
+                } else if (wf[1] != 0 && -val(q.Qo) > wf[1]) {
+                    q.Qo = T(-wf[1]);
+                    q.Qw = q.Qo * krw * M.wat_ro * WAT_M_RO * muo / (kro * roo * M.wat_mu);
+                    q.Qg = q.Qo * krg * rog * muo / (kro * roo * mug);
+                    q.PIw = q.PIo = q.PIg = T(0);
+                }
+                clamp_prod();
+                break;
+            case 1:   // constant Pwf, gas production (:268-318)
+                set_pi(S1pi);
+                pwf_rates(wf[0]);
+                clamp_prod();
+                if (cond) {
+                    const T s = sso1(1);
+                    if (val(s) != 0) q.Qg = -S1r * wf[1] * rog * krg / (mug * s);
+                    q.Qw = q.Qg * krw * M.wat_ro * WAT_M_RO * mug / (krg * rog * M.wat_mu);
+                    q.Qo = q.Qg * kro * roo * mug / (krg * rog * muo);
+                }
+                break;
+            case 2:   // constant Pwf, water injection (:320-350)
+                q.PIw = S1pi * M.wat_ro * WAT_M_RO * krw / M.wat_mu;
+                q.Qw = q.PIw * (wf[0] - p.Pw);
+                if (val(q.Qw) < 0 || nan_or(q.Qw)) { q.Qw = T(0); q.PIw = T(0); }
+                if (cond) {
+                    const T s = sso1(2);
+                    if (val(s) != 0) q.Qw = S1r * wf[1] * M.wat_ro * WAT_M_RO * krw / (M.wat_mu * s);
+                }
+                break;
+            case 3: {   // constant Pwf, gas injection (:352-400)
+                q.PIg = S1pi * ij[W_GD] * M.sgt[4 * (M.nsgt - 1) + 1] * ij[W_GF] / ij[W_GV];
+                q.Qg = q.PIg * (wf[0] - p.Pg);
+                q.PIo = q.PIg * ij[W_OF] / ij[W_GF];
+                q.Qo = q.PIo * (wf[0] - p.Pg);
+                if (val(q.Qo) < 0 || nan_or(q.Qo)) { q.Qo = T(0); q.PIo = T(0); }
+                if (val(q.Qg) < 0 || nan_or(q.Qg)) { q.Qg = T(0); q.PIg = T(0); }
+                q.inj = i;
+                if (cond) {
+                    const double s = sso1_geo();
+                    if (s != 0) q.Qg = T(S1r * wf[1] / s);
+                    if (s != 0) q.Qo = q.Qg * ij[W_OF] / ij[W_GF];
+                }
+                break;
+            }
+            case 4: {   // constant oil rate (:402-461)
+                const T s = sso1(0);
+                if (val(s) != 0) q.Qo = -S1r * wf[0] * roo * kro / (muo * s);
+                q.Qw = q.Qo * krw * M.wat_ro * WAT_M_RO * muo / (kro * roo * M.wat_mu);
+                q.Qg = q.Qo * krg * rog * muo / (kro * roo * mug);
+                if (cond) {
+                    set_pi(S1pi);
+                    pwf_rates(wf[0]);
+                    clamp_prod();
+                }
+                if (k == 1 && val(p.Po) < WELL_MIN_P) {
+                    q.Qw = q.Qo = q.Qg = T(0);
+                    q.PIw = q.PIo = q.PIg = T(0);
+                }
+                break;
+            }
+            case 5: {   // constant gas rate (:463-515)
+                const T s = sso1(1);
+                if (val(s) != 0) q.Qg = -S1r * wf[0] * rog * krg / (mug * s);
+                q.Qw = q.Qg * krw * M.wat_ro * WAT_M_RO * mug / (krg * rog * M.wat_mu);
+                q.Qo = q.Qg * kro * roo * mug / (krg * rog * muo);
+                if (cond) {
+                    set_pi(S1pi);
+                    pwf_rates(wf[0]);
+                    clamp_prod();
+                }
+                break;
+            }
+            case 6: {   // constant total production rate (:517-576)
+                T SSw = T(0), SSo = T(0), SSg = T(0), s = T(0);
+                for (int j = wi[3]; j <= wi[4]; j++) {
+                    for (int side = 0; side < 2; side++) {
+                        const Props<T> a = side == 0 ? own_at(j) : oth_at(j);
+                        const T mo_den = side == 0 ? p.mu[0] : T(val(pxc.mu[0]));
+                        const T S1 = (a.kr[0] / M.wat_mu) + (a.kr[1] / a.mu[0]) + (a.kr[2] / a.mu[1]);
+                        SSw += (a.kr[0] / M.wat_mu) / S1;
+                        SSo += (a.kr[1] / a.mu[0]) / S1;
+                        SSg += (a.kr[2] / a.mu[1]) / S1;
+                        s += a.ro[0] * (side == 0 ? S2own(j) : S2oth(j)) * a.kr[1] / mo_den;
+                    }
+                }
+                const T QoP = SSo * wf[0] / (SSw + SSo + SSg);
+                if (val(s) != 0) q.Qo = -S1r * QoP * roo * kro / (muo * s);
+                q.Qw = q.Qo * krw * M.wat_ro * WAT_M_RO * muo / (kro * roo * M.wat_mu);
+                q.Qg = q.Qo * krg * rog * muo / (kro * roo * mug);
+                if (cond) {
+                    set_pi(S1pi);
+                    pwf_rates(wf[0]);
+                    clamp_prod();
+                }
+                break;
+            }
+            case 7:   // constant water injection rate (:578-604): its rate
+                      // branch tests a sum that is never set, so only the
+                      // Pwf-limited branch ever injects
+                if (cond) {
+                    q.PIw = S1pi * M.wat_ro * WAT_M_RO * krw / M.wat_mu;
+                    q.Qw = q.PIw * (wf[0] - p.Pw);
+                    if (val(q.Qw) < 0 || nan_or(q.Qw)) { q.Qw = T(0); q.PIw = T(0); }
+                }
+                break;
+            case 8: {   // constant gas injection rate (:606-691)
+                const double s = sso1_geo();
+                if (s != 0) q.Qg = T(S1r * wf[0] / s);
+                if (s != 0) q.Qo = q.Qg * ij[W_OF] / ij[W_GF];
+                if (cond) {
+                    q.PIg = S1pi * ij[W_GD] * M.sgt[4 * (M.nsgt - 1) + 1] * ij[W_GF] / ij[W_GV];
+                    q.Qg = q.PIg * (wf[1] - p.Pg);
+                    q.PIo = q.PIg * ij[W_OF] / ij[W_GF];
+                    q.Qo = q.PIo * (wf[1] - p.Pg);
+                    if (val(q.Qo) < 0 || nan_or(q.Qo)) { q.Qo = T(0); q.PIo = T(0); }
+                    if (val(q.Qg) < 0 || nan_or(q.Qg)) { q.Qg = T(0); q.PIg = T(0); }
+                }
+                if (val(p.Po) > WELL_MAX_P) {
+                    q.Qw = q.Qo = q.Qg = T(0);
+                    q.PIw = q.PIo = q.PIg = T(0);
+                }
+                q.inj = i;
+                break;
+            }
+            default:
+                break;
+        }
+        (void)watf;
+    }
+    return q;
+}
+
 // ------------------------------------------------------------- residual
 template <class T>
 struct Neigh {
@@ -549,8 +1126,9 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
     const double sgn = k == 0 ? 1.0 : -1.0;   // fracture rows take the opposite sign
 
     const T dcl = T(lab == -1 ? 0.0 : 1.0), dcg = T(lab == 1 ? 0.0 : 1.0);
-    const T acc_pre_c = T(0);
-    (void)acc_pre_c;
+    // wells completed in this block (Petsc_Functions.cpp:164-697, :3505-4040)
+    const WellQ<T> wq = M.nw ? well_terms<T>(M, k, c, ix, iy, iz, p, Pp) : WellQ<T>();
+    const double* injx = wq.inj >= 0 ? &M.inj[(size_t)(2 * nc + 6) * wq.inj] : nullptr;
 
     // component balances i = 0..nc-2, then total hydrocarbon
     for (int i = 0; i < nc; i++) {
@@ -583,7 +1161,11 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
         }
         // convective matrix-fracture transfer
         if (!total) { acc_add(RS, sgn * vm.x[i] * convo); acc_add(RS, sgn * vm.y[i] * convg); }
-        else { acc_add(RS, sgn * convo); acc_add(RS, sgn * convg); }
+        else {
+            acc_add(RS, sgn * convo);
+            acc_add(RS, sgn * convg);
+            if (M.nw) { acc_add(RS, wq.Qo); acc_add(RS, wq.Qg); }   // :1143
+        }
         if (!total) {
             // diffusive matrix-fracture transfer (:5259-5313 matrix rows,
             // :740-806 fracture rows); interface compositions are zero
@@ -617,6 +1199,14 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
                 if (lm == 0 && lf == -1) { acc_add(RS, -(mg * dyo)); acc_add(RS, -(fg * fyo)); }
                 if (lm == 1 && lf == 0) { acc_add(RS, -(fg * fyo)); acc_add(RS, -(mo * dxo)); }
                 if (lm == -1 && lf == 0) { acc_add(RS, -(mg * dyo)); acc_add(RS, -(fo * fxo)); }
+            }
+            // wells: the block's own compositions, or an injector's stream
+            // (:787-808)
+            if (M.nw) {
+                const T oc = injx ? T(injx[i]) : v.x[i];
+                const T gc = injx ? T(injx[nc + i]) : v.y[i];
+                acc_add(RS, oc * wq.Qo);
+                acc_add(RS, gc * wq.Qg);
             }
             // inter-cell Fickian flux (:5346-5656): thickness-weighted
             // density x saturation over tortuosity/porosity (tortuosity 1
@@ -664,6 +1254,7 @@ void block_rows(const Model& M, int k, int c, const std::vector<Vars<T>>* V, con
         }
         acc_add(RS, sgn * (SI_C * M.wat_ro * WAT_M_RO * sf2 * krw_mf / M.wat_mu * Vb *
                      ((pf.Pw - pm.Pw) + (pm.pcw - pf.pcw))));
+        if (M.nw) acc_add(RS, wq.Qw);   // :1168
         out[r0 + nc] = g_absmode ? T(std::fabs(val(LS)) + val(RS)) : LS - RS;
     }
     // local constraints

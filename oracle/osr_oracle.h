@@ -120,6 +120,20 @@ typedef struct {
     const double *parachor;  /* [nc] */
 } or_fluid_ext;
 
+/* Wells (deck WELLS, Data_Input.cpp:1185-1330; run state simTime and
+ * WellCondition).  welli rows as Data_Input stores them: type 0..8, then the
+ * 0-based block Ix, Iy and the first and last completed layer; wellf rows of
+ * nc + 6 values in the type's order (Pwf or rate, limit, skin, rw, for types
+ * 3 and 8 the nc injection mole fractions, then start and end time, s). */
+typedef struct {
+    int32_t nw;
+    const int32_t *welli;    /* [nw*5] */
+    const double *wellf;     /* [nw*(nc+6)] */
+    const int8_t *condition; /* [nw] WellCondition, or NULL = all false */
+    double sim_time;         /* simTime, s */
+    double re_f;             /* Re_f, half fracture length, m */
+} or_wells;
+
 /* Grid, rock and run constants (deck GRID, DI/DJ/DK, POR/FPOR, PERM*, TOR,
  * REFP/CPOR/DCPOR, SIGMA, Lf, CDIF, RESTEMP, DT, WATERPROPS, SWT/SGT). */
 typedef struct {
@@ -130,6 +144,7 @@ typedef struct {
     const double *swt, *sgt;             /* [rows*4]: SAT, KR, KRO, PC */
     double refP, cpor, dcpor, sigma, lf, cdiff, T, dt, wat_ro, wat_mu;
     int32_t nx, ny, nz, const_diff, nswt, nsgt;
+    const or_wells *wells;               /* NULL = no wells */
 } or_grid;
 
 /* Per-cell state of both continua (n = nx*ny*nz, cell = iz*nx*ny+iy*nx+ix). */
@@ -159,6 +174,9 @@ int or_residual(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, con
 /* Same rows with every term taken by magnitude: the scale of each row. */
 int or_residual_scale(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *F);
 int or_props(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s, double *out);
+/* The grid's wells: Calc_Re (Minor_NR.cpp:2700-3249) re[nw] and the
+ * FlashInjection outputs (Init_Calcs.cpp:1161-1377) inj[nw][2nc+6]; returns nw. */
+int or_well_data(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, double *re, double *inj);
 /* IFK_Calc's kA/h (Init_Calcs.cpp:21-42): out [2][3][n]. */
 int or_iftran(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, double *out);
 int64_t or_jacobian(const or_fluid *f, const or_fluid_ext *fe, const or_grid *g, const or_state *s,

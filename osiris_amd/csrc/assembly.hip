@@ -57,6 +57,7 @@ struct AsmConsts {
     double pdij[NCMAX * NCMAX];   // pDij (Mass_Tran.cpp:41-45), 0 on the diagonal
     double ipdij[NCMAX * NCMAX];  // 1 / pDij, 0 on the diagonal
     double pdlast[NCMAX];         // pDij of the last k != j (used when x_j == 1)
+    double wk[NCMAX];             // Wilson exponent factor 5.37 (1 + omega_i)
     double swt[4 * MAXTAB], sgt[4 * MAXTAB];
     int nswt, nsgt;
     double refP, cpor, dcpor, dt, wat_ro, wat_mu, sigma, lf, cdiff;
@@ -84,6 +85,15 @@ struct AsmArgs {
     double* props;           // [NF][2n]
     const int64_t* boff;     // [2n+1]
     int xcd;                 // 1: XCD-contiguous workgroup order (wg_index)
+    const double* perm_f;    // [3n] fracture permeabilities
+    // wells (osr_asm_set_wells; nw = 0: none)
+    int nw;
+    const double* wsc;       // [2] simTime, Re_f
+    const int* wi;           // [nw][5] type, Ix, Iy, first and last layer
+    const double* wf;        // [nw][nc + 6]
+    const int8_t* wcond;     // [nw] WellCondition
+    const double* wre;       // [nw] Calc_Re of the well's block
+    const double* winj;      // [nw][2nc + 6] FlashInjection of types 3 and 8
 };
 
 // Logical workgroup index.  The dispatcher deals workgroups round-robin to
@@ -211,6 +221,23 @@ struct PropsT {
     T D[2][NC];
 };
 
+// slope (s - s1) + v1 with the product rounded before the sum, as the
+// reference evaluates it: not fused into one FMA, so a saturation on the
+// segment's left node gives that node's value exactly (a zero relative
+// permeability stays zero, where the reference divides by it)
+__device__ __forceinline__ double seg_rn(double slope, double s, double s1, double v1) {
+#pragma clang fp contract(off)
+    return slope * (s - s1) + v1;
+}
+template <int W>
+__device__ __forceinline__ DualV<W> seg_rn(double slope, const DualV<W>& s, double s1, double v1) {
+    DualV<W> r;
+    r.v = seg_rn(slope, s.v, s1, v1);
+#pragma unroll
+    for (int q = 0; q < W; q++) r.d[q] = slope * s.d[q];
+    return r;
+}
+
 template <class T>
 __device__ __forceinline__ void table(const double* tb, int nr, const T& s, T& kr, T& kro, T& pc) {
     int r = 0;
@@ -227,9 +254,9 @@ __device__ __forceinline__ void table(const double* tb, int nr, const T& s, T& k
         // slope as a quotient, as the reference writes it (Do_Cycle.cpp:183-185),
         // so a saturation on a table node gives the node value exactly
         const double s0 = tb[4 * (r - 1)], s1 = tb[4 * r];
-        kr = (tb[4 * r + 1] - tb[4 * (r - 1) + 1]) / (s1 - s0) * (s - s1) + tb[4 * r + 1];
-        kro = (tb[4 * r + 2] - tb[4 * (r - 1) + 2]) / (s1 - s0) * (s - s1) + tb[4 * r + 2];
-        pc = (tb[4 * r + 3] - tb[4 * (r - 1) + 3]) / (s1 - s0) * (s - s1) + tb[4 * r + 3];
+        kr = seg_rn((tb[4 * r + 1] - tb[4 * (r - 1) + 1]) / (s1 - s0), s, s1, tb[4 * r + 1]);
+        kro = seg_rn((tb[4 * r + 2] - tb[4 * (r - 1) + 2]) / (s1 - s0), s, s1, tb[4 * r + 2]);
+        pc = seg_rn((tb[4 * r + 3] - tb[4 * (r - 1) + 3]) / (s1 - s0), s, s1, tb[4 * r + 3]);
     }
 }
 
@@ -2285,6 +2312,7 @@ __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
 }  // namespace osr
 
+#include "wells_dev.hpp"
 #include "refjac_dev.hpp"
 
 // ============================================================== host side
@@ -2310,6 +2338,15 @@ struct osr_asm {
     void* scan_tmp = nullptr;
     size_t scan_bytes = 0;
     int64_t nrows = -1;
+    // wells (osr_asm_set_wells): host cell sizes for Calc_Re; device copies
+    // of the well table, Calc_Re and FlashInjection per well, the completed
+    // blocks, [simTime, Re_f]; kept while counts fit, so a captured graph
+    // reads the values of the last call
+    std::vector<double> hdx, hdy;
+    int nw = 0, nwb = 0, cap_w = 0, cap_b = 0;
+    int *d_wi = nullptr, *d_wblk = nullptr;
+    double *d_wf = nullptr, *d_wre = nullptr, *d_winj = nullptr, *d_wsc = nullptr;
+    int8_t* d_wcond = nullptr;
 };
 
 namespace {
@@ -2351,7 +2388,24 @@ osr::AsmArgs make_args(osr_asm* a, const osr_state_dev* s) {
     A.props = a->props;
     A.boff = a->boff;
     A.xcd = a->ctx->xcd_remap;
+    A.perm_f = a->perm_f;
+    A.nw = a->nw;
+    A.wsc = a->d_wsc;
+    A.wi = a->d_wi;
+    A.wf = a->d_wf;
+    A.wcond = a->d_wcond;
+    A.wre = a->d_wre;
+    A.winj = a->d_winj;
     return A;
+}
+
+// the well kernels after the kernels that write the completed blocks' rows
+template <int NC>
+int wells_res(osr_asm* a, const osr::AsmArgs& A, double* F, hipStream_t st) {
+    if (a->nw == 0 || a->nwb == 0) return OSR_OK;
+    hipLaunchKernelGGL(osr::k_well_res<NC>, dim3((a->nwb + 63) / 64), dim3(64), 0, st, A, (const int*)a->d_wblk,
+                       a->nwb, F);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
 template <int NC>
@@ -2384,7 +2438,8 @@ int residual_nc(osr_asm* a, const osr_state_dev* s, double* F, hipStream_t st) {
     if (rc) return rc;
     const int nb = 2 * a->n;
     hipLaunchKernelGGL(osr::k_residual<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, A, F);
-    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+    if (hipGetLastError() != hipSuccess) return OSR_E_HIP;
+    return wells_res<NC>(a, A, F, st);
 }
 
 template <int NC>
@@ -2414,6 +2469,9 @@ int jacobian_nc(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, int32
     const int64_t gn = (nb * 7 * MAXC + 255) / 256;
     hipLaunchKernelGGL(osr::k_jac_nb<NC>, dim3((unsigned)gn), dim3(256), 0, st, A, (const double*)a->dprops,
                        rowptr, cols, vals);
+    if (a->nw > 0 && a->nwb > 0)
+        hipLaunchKernelGGL(osr::k_well_jac<NC>, dim3((unsigned)(((int64_t)a->nwb * MAXC + 63) / 64)), dim3(64), 0, st,
+                           A, (const int*)a->d_wblk, a->nwb, (const double*)a->dprops, rowptr, vals);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -2516,6 +2574,9 @@ int reduced_direct_nc(osr_asm* a, const osr_state_dev* s, const double* F, const
     hipLaunchKernelGGL(osr::k_red_own<NC>, dim3((unsigned)((nb * (NP + 1) + wo - 1) / wo)), dim3(wo), 0, st, A,
                        (const double*)a->dred, browptr, bcols,
                        bvals, a->rpart);
+    if (a->nw > 0 && a->nwb > 0)
+        hipLaunchKernelGGL(osr::k_well_red<NC>, dim3((unsigned)(((int64_t)a->nwb * (NP + 1) + 63) / 64)), dim3(64), 0,
+                           st, A, (const int*)a->d_wblk, a->nwb, (const double*)a->dred, browptr, bvals, a->rpart);
     hipLaunchKernelGGL(osr::k_red_rhs<NC>, dim3(g1), dim3(256), 0, st, A, F, (const double*)a->rpart, brhs);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
@@ -2527,6 +2588,109 @@ int expand_nc(osr_asm* a, const osr_state_dev* s, const double* S, const double*
     const int64_t nb = 2 * (int64_t)a->n;
     hipLaunchKernelGGL(osr::k_reduce_expand<NC>, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, A, S, t, dp,
                        du);
+    return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+// Calc_Re (Minor_NR.cpp:2700-3249) of a well at the centre of its block
+// (X_posi = Y_posi = 0, where the reference places it): with face factors
+// T_j = dx/dy, dy/dx, dx/dy, dy/dx and distances r_j = dy, dx, dy, dx to the
+// face neighbours j = 1..4 (+y, +x, -y, -x) and images r'_aj of the well in
+// the reservoir boundaries, at distances a,
+//     Re = (e^{-2 pi} prod_j r_j^T_j prod_a (r'_aj / a)^T_j)^(1 / sum T),
+// 0.2079 d on an interior square block (Peaceman's e^{-pi/2} d).  The
+// diagonal neighbours carry T = 0 at the centre and drop out.  Each
+// boundary case is one row of the table below, applied in the reference's
+// order on shared factors: on a one-block-wide grid several rows apply in
+// turn, and the later ones see what the earlier ones left, as there.
+struct ReImage {
+    int8_t a, j, mx, my;   // image a of face j at (mx dx, my dy)
+};
+struct ReRow {
+    int8_t kind;           // 0 interior, 1 side (faces in set cleared), 2 corner (faces in set kept)
+    int8_t set[2], nset;
+    int8_t aj[3][2];       // distance to image a = 2, 3, 4 as (mx, my)
+    ReImage img[6];
+    int8_t nimg;
+};
+const ReRow RE_ROWS[9] = {
+    {0, {0, 0}, 0, {{0, 0}, {0, 0}, {0, 0}}, {}, 0},
+    {1, {1, 0}, 1, {{0, 1}, {0, 0}, {0, 0}}, {{2, 2, 1, 1}, {2, 3, 0, 2}, {2, 4, 1, 1}}, 3},   // Iy = ny - 1
+    {1, {3, 0}, 1, {{0, 1}, {0, 0}, {0, 0}}, {{2, 2, 1, 1}, {2, 1, 0, 2}, {2, 4, 1, 1}}, 3},   // Iy = 0
+    {1, {2, 0}, 1, {{1, 0}, {0, 0}, {0, 0}}, {{2, 1, 1, 1}, {2, 4, 2, 0}, {2, 3, 1, 1}}, 3},   // Ix = nx - 1
+    {1, {4, 0}, 1, {{1, 0}, {0, 0}, {0, 0}}, {{2, 1, 1, 1}, {2, 2, 2, 0}, {2, 3, 1, 1}}, 3},   // Ix = 0
+    {2, {2, 3}, 2, {{0, 1}, {1, 1}, {1, 0}},                                                  // (0, ny - 1)
+     {{2, 2, 1, 1}, {2, 3, 0, 2}, {3, 2, 2, 1}, {3, 3, 1, 2}, {4, 2, 2, 0}, {4, 3, 1, 1}}, 6},
+    {2, {1, 2}, 2, {{0, 1}, {1, 1}, {1, 0}},                                                  // (0, 0)
+     {{2, 2, 1, 1}, {2, 1, 0, 2}, {3, 2, 2, 1}, {3, 1, 1, 2}, {4, 2, 2, 0}, {4, 1, 1, 1}}, 6},
+    {2, {3, 4}, 2, {{1, 0}, {1, 1}, {0, 1}},                                                  // (nx - 1, ny - 1)
+     {{2, 4, 2, 0}, {2, 3, 1, 1}, {3, 4, 2, 1}, {3, 3, 1, 2}, {4, 3, 0, 2}, {4, 4, 1, 1}}, 6},
+    {2, {1, 4}, 2, {{1, 0}, {1, 1}, {0, 1}},                                                  // (nx - 1, 0)
+     {{2, 1, 1, 1}, {2, 4, 2, 0}, {3, 4, 2, 1}, {3, 1, 1, 2}, {4, 1, 0, 2}, {4, 4, 1, 1}}, 6},
+};
+
+double calc_re(int nx, int ny, double dx, double dy, int ix, int iy) {
+    auto len = [&](int mx, int my) {
+        const double u = mx * dx, v = my * dy;
+        return std::sqrt(u * u + v * v);
+    };
+    double T[5] = {0, dx / dy, dy / dx, dx / dy, dy / dx};
+    double r[5] = {0, dy, dx, dy, dx};
+    double rij[5][5] = {}, aj[5] = {};
+    double bsum = 0, iz = 1, re = 0;
+    const double e2pi = std::exp(-2 * 3.1415926535897932384626433);
+    const bool inx = ix != 0 && ix != nx - 1, iny = iy != 0 && iy != ny - 1;
+    const bool apply[9] = {0 < ix && ix < nx - 1 && 0 < iy && iy < ny - 1,
+                           iy == ny - 1 && inx, iy == 0 && inx, ix == nx - 1 && iny, ix == 0 && iny,
+                           ix == 0 && iy == ny - 1, ix == 0 && iy == 0, ix == nx - 1 && iy == ny - 1,
+                           ix == nx - 1 && iy == 0};
+    for (int b = 0; b < 9; b++) {
+        if (!apply[b]) continue;
+        const ReRow& R = RE_ROWS[b];
+        for (int a = 0; a < 3; a++)
+            if (R.aj[a][0] || R.aj[a][1]) aj[a + 2] = len(R.aj[a][0], R.aj[a][1]);
+        for (int m = 0; m < R.nimg; m++) rij[R.img[m].a][R.img[m].j] = len(R.img[m].mx, R.img[m].my);
+        for (int j = 1; j <= 4; j++) {
+            bool in = false;
+            for (int m = 0; m < R.nset; m++) in = in || R.set[m] == j;
+            if ((R.kind == 1 && in) || (R.kind == 2 && !in)) {
+                T[j] = 0;
+                r[j] = 1;
+                rij[2][j] = 1;
+                if (R.kind == 2) rij[3][j] = rij[4][j] = 1;
+            }
+        }
+        if (R.kind == 0)
+            for (int j = 1; j <= 4; j++) iz *= std::pow(r[j], T[j]);
+        for (int j = 1; j <= 4; j++) bsum += T[j];
+        bsum = 1 / bsum;
+        if (R.kind == 1)
+            for (int j = 1; j <= 4; j++) iz *= std::pow(r[j], T[j]) * std::pow(rij[2][j] / aj[2], T[j]);
+        if (R.kind == 2) {
+            iz = 1;
+            for (int j = 1; j <= 4; j++) {
+                double jz = 1;
+                for (int a = 2; a <= 4; a++) jz *= std::pow(rij[a][j] / aj[a], T[j]);
+                iz *= std::pow(r[j], T[j]) * jz;
+            }
+        }
+        re = std::pow(e2pi * iz, bsum);
+    }
+    return re;
+}
+
+template <class Tv>
+int ensure(Tv** p, int need, int cap) {
+    if (*p && need <= cap) return OSR_OK;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    OSR_HIP(hipMalloc((void**)p, (size_t)(need > 0 ? need : 1) * sizeof(Tv)));
+    return OSR_OK;
+}
+
+template <int NC>
+int flash_injection_nc(osr_asm* a, hipStream_t st) {
+    hipLaunchKernelGGL(osr::k_flash_injection<NC>, dim3((a->nw + 63) / 64), dim3(64), 0, st,
+                       (const osr::AsmConsts*)a->dK, a->nw, (const int*)a->d_wi, (const double*)a->d_wf, a->d_winj);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -2563,6 +2727,8 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
     a->ctx = ctx;
     a->nx = g->nx; a->ny = g->ny; a->nz = g->nz; a->n = (int)n64; a->nc = ctx->nc;
     const int n = a->n, nc = a->nc;
+    a->hdx.assign(g->dx, g->dx + g->nx);
+    a->hdy.assign(g->dy, g->dy + g->ny);
     osr::AsmConsts& K = a->hK;
     K.nc = nc;
     K.eos = ctx->eos;
@@ -2583,6 +2749,7 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
         K.v53[i] = std::pow(K.vcm3[i], 5.0 / 3.0);
         K.v23[i] = std::pow(K.vcm3[i], 2.0 / 3.0);
         K.par[i] = fe->parachor[i];
+        K.wk[i] = F.wk[i];
         const double tr = g->T / F.Tc[i];
         const double gam = std::pow(F.Tc[i], 1.0 / 6) / (std::sqrt(fe->MW[i]) * std::pow(F.Pc[i] / 101325, 2.0 / 3));
         K.mu0[i] = (4.61 * std::pow(tr, 0.618) - 2.04 * std::exp(-0.449 * tr) + 1.94 * std::exp(-4.058 * tr) + 0.1) *
@@ -2671,10 +2838,84 @@ int osr_asm_create(osr_ctx* ctx, const osr_fluid_ext* fe, const osr_grid* g, osr
     return OSR_OK;
 }
 
+int osr_asm_set_wells(osr_asm* a, const osr_wells* w, void* stream) {
+    if (!a || !w || w->nw < 0) return OSR_E_ARG;
+    const int nw = w->nw, nc = a->nc, n = a->n;
+    if (nw == 0) {
+        a->nw = 0;
+        a->nwb = 0;
+        return OSR_OK;
+    }
+    if (!w->welli || !w->wellf || !(w->re_f >= 0)) return OSR_E_ARG;
+    std::vector<double> re(nw);
+    std::vector<int> blocks;
+    std::vector<char> seen(2 * (size_t)n, 0);
+    for (int i = 0; i < nw; i++) {
+        const int32_t* wi = w->welli + 5 * i;
+        const double* wf = w->wellf + (size_t)(nc + 6) * i;
+        if (wi[0] < 0 || wi[0] > 8 || wi[1] < 0 || wi[1] >= a->nx || wi[2] < 0 || wi[2] >= a->ny || wi[3] < 0 ||
+            wi[4] < wi[3] || wi[4] >= a->nz || !(wf[3] > 0))
+            return OSR_E_ARG;
+        re[i] = calc_re(a->nx, a->ny, a->hdx[wi[1]], a->hdy[wi[2]], wi[1], wi[2]);
+        for (int k = 0; k < 2; k++)
+            for (int z = wi[3]; z <= wi[4]; z++) {
+                const int blk = k * n + (z * a->ny + wi[2]) * a->nx + wi[1];
+                if (!seen[blk]) {
+                    seen[blk] = 1;
+                    blocks.push_back(blk);
+                }
+            }
+    }
+    std::sort(blocks.begin(), blocks.end());
+    const int nwb = (int)blocks.size();
+    std::vector<int8_t> cond(nw, 0);
+    if (w->condition)
+        for (int i = 0; i < nw; i++) cond[i] = w->condition[i] ? 1 : 0;
+    const double sc[2] = {w->sim_time, w->re_f};
+    OSR_HIP_RET(ensure(&a->d_wi, 5 * nw, 5 * a->cap_w));
+    OSR_HIP_RET(ensure(&a->d_wf, (nc + 6) * nw, (nc + 6) * a->cap_w));
+    OSR_HIP_RET(ensure(&a->d_wcond, nw, a->cap_w));
+    OSR_HIP_RET(ensure(&a->d_wre, nw, a->cap_w));
+    OSR_HIP_RET(ensure(&a->d_winj, (2 * nc + 6) * nw, (2 * nc + 6) * a->cap_w));
+    a->cap_w = std::max(a->cap_w, nw);
+    OSR_HIP_RET(ensure(&a->d_wblk, nwb, a->cap_b));
+    a->cap_b = std::max(a->cap_b, nwb);
+    OSR_HIP_RET(ensure(&a->d_wsc, 2, a->d_wsc ? 2 : 0));
+    // the previous call's kernels may still read the tables: copy in stream order
+    const hipStream_t st = (hipStream_t)stream;
+    OSR_HIP(hipStreamSynchronize(st));
+    OSR_HIP(hipMemcpy(a->d_wi, w->welli, 5 * (size_t)nw * sizeof(int), hipMemcpyHostToDevice));
+    OSR_HIP(hipMemcpy(a->d_wf, w->wellf, (size_t)(nc + 6) * nw * sizeof(double), hipMemcpyHostToDevice));
+    OSR_HIP(hipMemcpy(a->d_wcond, cond.data(), (size_t)nw, hipMemcpyHostToDevice));
+    OSR_HIP(hipMemcpy(a->d_wre, re.data(), (size_t)nw * sizeof(double), hipMemcpyHostToDevice));
+    OSR_HIP(hipMemcpy(a->d_wblk, blocks.data(), (size_t)nwb * sizeof(int), hipMemcpyHostToDevice));
+    OSR_HIP(hipMemcpy(a->d_wsc, sc, sizeof(sc), hipMemcpyHostToDevice));
+    OSR_HIP(hipMemset(a->d_winj, 0, (size_t)(2 * nc + 6) * nw * sizeof(double)));
+    a->nw = nw;
+    a->nwb = nwb;
+    int rc = OSR_OK;
+#define CALL(NC) flash_injection_nc<NC>(a, st)
+    OSR_NC_SWITCH_RC(nc)
+#undef CALL
+    if (rc) return rc;
+    OSR_HIP(hipStreamSynchronize(st));
+    return OSR_OK;
+}
+
+int osr_asm_well_data(osr_asm* a, double* re, double* inj) {
+    if (!a) return OSR_E_ARG;
+    if (a->nw == 0) return OSR_OK;
+    if (re) OSR_HIP(hipMemcpy(re, a->d_wre, (size_t)a->nw * sizeof(double), hipMemcpyDeviceToHost));
+    if (inj)
+        OSR_HIP(hipMemcpy(inj, a->d_winj, (size_t)a->nw * (2 * a->nc + 6) * sizeof(double), hipMemcpyDeviceToHost));
+    return OSR_OK;
+}
+
 int osr_asm_destroy(osr_asm* a) {
     if (!a) return OSR_E_ARG;
     void* ptrs[] = {a->dK, a->dx, a->dy, a->dz, a->h, a->phi, a->perm_m, a->iftran, a->tor,
-                    a->props, a->rho_tmp, a->boff, a->bs, a->perm_f, a->pslots, a->bcnt, a->scan_tmp};
+                    a->props, a->rho_tmp, a->boff, a->bs, a->perm_f, a->pslots, a->bcnt, a->scan_tmp,
+                    a->d_wi, a->d_wblk, a->d_wf, a->d_wre, a->d_winj, a->d_wsc, a->d_wcond};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (!a->ws_external) {
@@ -2821,6 +3062,7 @@ int osr_jacobian_ref_device(osr_asm* a, const osr_state_dev* s, const int64_t* r
     int rc = check_state(a, s);
     if (rc) return rc;
     if (!rowptr || !cols || !vals) return OSR_E_ARG;
+    if (a->nw > 0) return OSR_E_ARG;   // the reference rules of the well columns are not restated
 #define CALL(NC) rj_jacobian_nc<NC>(a, s, rowptr, cols, vals, (hipStream_t)stream)
     OSR_NC_SWITCH(a->nc, CALL)
 #undef CALL

@@ -83,6 +83,7 @@ template <int W> __device__ __forceinline__ DualV<W> operator/(double a, const D
 }
 template <int W> __device__ __forceinline__ DualV<W>& operator+=(DualV<W>& a, const DualV<W>& b) { return a = a + b; }
 template <int W> __device__ __forceinline__ DualV<W>& operator-=(DualV<W>& a, const DualV<W>& b) { return a = a - b; }
+template <int W> __device__ __forceinline__ DualV<W>& operator+=(DualV<W>& a, double b) { a.v += b; return a; }
 
 #define OSR_DUAL_CMP(op)                                                                                     \
     template <int W> __device__ __forceinline__ bool operator op(const DualV<W>& a, const DualV<W>& b) { return a.v op b.v; } \

@@ -207,7 +207,7 @@ __device__ __forceinline__ void psat_cell(const FluidDev* __restrict__ F, const 
     } while (fabs(SXm - 1) / k > SXMTOL);
     if (sumK / k < TRIV_K) bad = true;
     double result;
-    if (!bad && (Pm - Pm) == 0) {
+    if (!bad && isfinite(Pm)) {   // the reference's (Pm - Pm) == 0, robust to FMA contraction
         result = Pm;
     } else {
         // ---- stage 2: Newton in (Xm, P)

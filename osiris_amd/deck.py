@@ -20,8 +20,8 @@ longest numeric prefix, 0 if none).  Keywords read, with their units:
   ABSTOL, RELTOL, DIVTOL, MAXITERS, SOLVERTYPE (defaults 1e-50, 1e-18, 1e3,
   5000, bcgs, :232-266); DBS option pairs; INITCOND 0 (per-cell arrays with
   n*v repeats, ECLStar :1941-1975), 1 (per layer) or 3 (uniform) with
-  IPRESS, FIPRESS (Pa), IWS, FIWS, IGC, FIGC; WELLS (read and kept; wells
-  are outside this path).
+  IPRESS, FIPRESS (Pa), IWS, FIWS, IGC, FIGC; WELLS (wells_table turns them
+  into the table the assembler completes).
 
 Refused with DeckError where the reference's own reading is undefined:
 JVAR distributions (its loops test the wrong index, :150, 190, 300, 347,
@@ -365,9 +365,8 @@ def _initial(r, nx, ny, nz, nc):
 
 
 def _wells(r, nwell, nc):
-    """WELLS (Data_Input.cpp:1180-1334): kept as data -- type, 0-based block
-    indices (i, j, k first, k last), and the type's values with the two
-    times in seconds.  The well terms of the residual are outside this path."""
+    """WELLS (Data_Input.cpp:1180-1334): type, 0-based block indices (i, j,
+    k first, k last), and the type's values with the two times in seconds."""
     if not r.search("WELLS"):
         return []
     r.word()
@@ -384,6 +383,22 @@ def _wells(r, nwell, nc):
         wells.append({"type": wtype, "i": ints[1], "j": ints[2], "k_first": ints[3], "k_last": ints[4],
                       "values": vals})
     return wells
+
+
+def wells_table(deck, sim_time, condition=None):
+    """The deck's wells as a synth.Wells (welli rows type, Ix, Iy, first and
+    last layer; wellf rows of nc + 6 values, the type's values first) at
+    simTime = sim_time s, with WellCondition `condition` (default all false,
+    as Init_Calcs.cpp:19 starts them) and the deck's Re_f."""
+    nc = deck.fluid.nc
+    nw = len(deck.wells)
+    wi = np.zeros((nw, 5), np.int32)
+    wf = np.zeros((nw, nc + 6))
+    for i, w in enumerate(deck.wells):
+        wi[i] = (w["type"], w["i"], w["j"], w["k_first"], w["k_last"])
+        wf[i, :len(w["values"])] = w["values"]
+    cond = np.zeros(nw, np.int8) if condition is None else np.asarray(condition, np.int8)
+    return synth.Wells(wi, wf, cond, sim_time=float(sim_time), re_f=float(deck.run["re_f"]))
 
 
 def initial_state(deck, flash_fn):

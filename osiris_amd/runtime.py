@@ -331,6 +331,11 @@ class _Grid(C.Structure):
                 ("const_diff", C.c_int32), ("nswt", C.c_int32), ("nsgt", C.c_int32)]
 
 
+class _Wells(C.Structure):
+    _fields_ = [("nw", C.c_int32), ("welli", C.POINTER(C.c_int32)), ("wellf", _dp),
+                ("condition", C.POINTER(C.c_int8)), ("sim_time", C.c_double), ("re_f", C.c_double)]
+
+
 class _StateDev(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in
                 ("label_m", "label_f", "Po_m", "Po_f", "x_m", "y_m", "x_f", "y_f", "S_m", "S_f",
@@ -365,6 +370,8 @@ def _bind_asm(L):
     L.osr_jacobian_bsr_reduced_direct_device.argtypes = [C.c_void_p, C.POINTER(_StateDev)] + [C.c_void_p] * 8
     L.osr_asm_workspace_size.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
     L.osr_asm_set_workspace.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    L.osr_asm_set_wells.argtypes = [C.c_void_p, C.POINTER(_Wells), C.c_void_p]
+    L.osr_asm_well_data.argtypes = [C.c_void_p, _dp, _dp]
     L._asm_bound = True
 
 
@@ -398,6 +405,40 @@ class Assembler:
         _check(L.osr_asm_create(ctx.h, C.byref(fe), C.byref(g), C.byref(h)), "osr_asm_create")
         self.h = h
         self.nrows = None
+        self.nw = 0
+        if getattr(grid, "wells", None) is not None and grid.wells.nw > 0:
+            self.set_wells(grid.wells)
+
+    def set_wells(self, wells, stream=None):
+        """Complete the wells (a synth.Wells, or None for none) in this grid:
+        the residual, Jacobian and reduced-system calls add their terms.
+        Call again when sim_time or condition change."""
+        nw = 0 if wells is None else wells.nw
+        if nw == 0:
+            w = _Wells(0, None, None, None, 0.0, 0.0)
+            _check(lib().osr_asm_set_wells(self.h, C.byref(w), _stream_ptr(stream)), "osr_asm_set_wells")
+            self.nw = 0
+            return
+        wi = np.ascontiguousarray(wells.welli, dtype=np.int32)
+        wf = np.ascontiguousarray(wells.wellf, dtype=np.float64)
+        if wi.shape != (nw, 5) or wf.shape != (nw, self.nc + 6):
+            raise ValueError("wells: welli must be [nw, 5] and wellf [nw, nc + 6]")
+        cond = np.zeros(nw, np.int8) if wells.condition is None else \
+            np.ascontiguousarray(wells.condition, dtype=np.int8)
+        w = _Wells(nw, wi.ctypes.data_as(C.POINTER(C.c_int32)), wf.ctypes.data_as(_dp),
+                   cond.ctypes.data_as(C.POINTER(C.c_int8)), float(wells.sim_time), float(wells.re_f))
+        _check(lib().osr_asm_set_wells(self.h, C.byref(w), _stream_ptr(stream)), "osr_asm_set_wells")
+        self.nw = nw
+
+    def well_data(self):
+        """(re [nw], inj [nw, 2nc + 6]) of the last set_wells: Calc_Re and
+        the FlashInjection outputs (x, y, oil/gas fraction, density, viscosity)."""
+        re = np.zeros(self.nw)
+        inj = np.zeros((self.nw, 2 * self.nc + 6))
+        if self.nw:
+            _check(lib().osr_asm_well_data(self.h, re.ctypes.data_as(_dp), inj.ctypes.data_as(_dp)),
+                   "osr_asm_well_data")
+        return re, inj
 
     def close(self):
         if getattr(self, "h", None):

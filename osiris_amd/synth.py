@@ -123,6 +123,24 @@ SWT_TEST_DAT = np.array([
 
 
 @dataclass
+class Wells:
+    """Wells as the reference keeps them (deck WELLS, Data_Input.cpp:1185-1330):
+    welli[w] = (type 0..8, Ix, Iy, first layer, last layer), 0-based blocks;
+    wellf[w] = the type's nc + 6 values (Pwf or rate, limit, skin, rw, for
+    types 3 and 8 the injection mole fractions, then start and end time in
+    s); condition = WellCondition; sim_time = simTime, s; re_f = Re_f, m."""
+    welli: np.ndarray          # [nw, 5] int32
+    wellf: np.ndarray          # [nw, nc + 6]
+    condition: np.ndarray      # [nw] int8
+    sim_time: float = 0.0
+    re_f: float = 0.0
+
+    @property
+    def nw(self):
+        return int(self.welli.shape[0])
+
+
+@dataclass
 class Grid:
     nx: int
     ny: int
@@ -148,6 +166,7 @@ class Grid:
     dt: float = 86400.0
     wat_ro: float = 1000.0
     wat_mu: float = 1.0
+    wells: "Wells" = None
 
     @property
     def n(self):

@@ -64,3 +64,53 @@ def column_variable(st, nc, n, bsize, boff, col):
     if lab != 1:
         order += [("S" + sfx, (2, c))]
     return order[j]
+
+
+# ------------------------------------------------------------------ wells
+def wrow(nc, t, ix, iy, k0, k1, a, b, skin=0.0, rw=0.1, z=None, t0=0.0, t1=1e9):
+    """One well: (welli row, wellf row) in the reference's order."""
+    f = np.zeros(nc + 6)
+    f[:4] = a, b, skin, rw
+    if t in (3, 8):
+        f[4:4 + nc] = z
+        f[nc + 4], f[nc + 5] = t0, t1
+    else:
+        f[4], f[5] = t0, t1
+    return [t, ix, iy, k0, k1], f
+
+
+def with_wells(grid, rows, sim_time=3600.0, re_f=50.0, cond=None):
+    g = copy.copy(grid)
+    wi = np.array([r[0] for r in rows], np.int32)
+    wf = np.array([r[1] for r in rows], float)
+    c = np.zeros(len(rows), np.int8) if cond is None else np.asarray(cond, np.int8)
+    g.wells = synth.Wells(wi, wf, c, sim_time=sim_time, re_f=re_f)
+    return g
+
+
+def bal_rows(fl, st, off, blk, n):
+    """Global rows of block blk's nc + 1 balance rows (components, total, water)."""
+    lab = (st.label_m if blk < n else st.label_f)[blk % n]
+    return off[blk] + (fl.nc if lab == 0 else 0) + np.arange(fl.nc + 1)
+
+
+def cell(grid, ix, iy, iz):
+    return (iz * grid.ny + iy) * grid.nx + ix
+
+
+def column_state(st, grid, ix, iy, **vals):
+    """A copy of the state with the cells of column (ix, iy) of both continua
+    set (a tuple value: (matrix, fracture))."""
+    s = copy.deepcopy(st)
+    for iz in range(grid.nz):
+        c = cell(grid, ix, iy, iz)
+        for kk, k in enumerate(("m", "f")):
+            for name, v in vals.items():
+                if isinstance(v, tuple):
+                    v = v[kk]
+                arr = getattr(s, f"{name}_{k}")
+                if arr.ndim == 1:
+                    arr[c] = v
+                else:
+                    arr[:, c] = v
+    return s
