@@ -265,8 +265,9 @@ typedef struct osr_wells {
  * whenever simTime or WellCondition change (a call with the same number of
  * wells and completed blocks keeps the device tables, so a graph captured
  * earlier reads the new values); nw = 0 removes the wells.  The reference-
- * mode Jacobian (osr_jacobian_ref_*) refuses an assembler with wells
- * (OSR_E_ARG). */
+ * mode Jacobian (osr_jacobian_ref_device) takes the reference's own well
+ * entries instead: PCoeff x the productivity indices in the block's P
+ * column and, for a producer, -Qo, -Qg in its x_i, y_i columns. */
 int osr_asm_set_wells(osr_asm *a, const osr_wells *w, void *stream);
 
 /* Per-well results of the last osr_asm_set_wells (host outputs, either may

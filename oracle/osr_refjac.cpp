@@ -236,6 +236,7 @@ struct RowOut {
 
 struct Ctx {
     const Model& M;
+    const Sweep* S = nullptr;     // property sweep (the well terms read it)
     const RefCells* R[2];
     std::vector<RefFace> F[2];   // [c*3 + d]: face to the +d neighbour
     std::vector<int64_t> boff;   // preConRow
@@ -662,6 +663,24 @@ void balance_row(const Ctx& X, int kc, int c, int i, bool total, RowOut& out) {
 
     // ------------------------------------------------ own and other P, S
     out.add(jacCol + 2 * nc - reCol, X.pcoeff * (w.dLp + dRsidedp));
+    // wells (matrix :32543, total :35425; fracture :15033, :17868): the
+    // productivity indices in the P column -- the oil part on the block's
+    // own x_i, the gas part on the injection stream's y_i in an injector's
+    // block -- and, for a producer, -Qo, -Qg in the x_i, y_i columns
+    // (:32808-32813; zero for injectors, and in the total row)
+    WellQ<double> wq;
+    if (M.nw) wq = well_terms<double>(M, kc, c, ix, iy, iz, X.S->P[kc][c], X.S->P);
+    if (M.nw) {
+        if (total) out.add(jacCol + 2 * nc - reCol, X.pcoeff * (wq.PIo + wq.PIg));
+        else {
+            const double gc = wq.inj >= 0 ? M.inj[(size_t)(2 * nc + 6) * wq.inj + nc + i] : s.X(c, i, 1);
+            out.add(jacCol + 2 * nc - reCol, X.pcoeff * (s.X(c, i, 0) * wq.PIo + gc * wq.PIg));
+            if (wq.inj < 0) {
+                if (lab != -1) out.add(jacCol + i, -wq.Qo);
+                if (lab != 1) out.add(jacCol + nc + i - reCol, -wq.Qg);
+            }
+        }
+    }
     int reCol3;
     if (lab != -1) {
         out.add(jacCol + 2 * nc + 1 - reCol, -dRsidedSw);
@@ -878,6 +897,10 @@ void water_row(const Ctx& X, int kc, int c, RowOut& out) {
     for (int q = 0; q < 6; q++)
         if (has[q]) dRsidedp += X.F[kc][(size_t)G[q].a * 3 + G[q].d].W;
     out.add(jacCol + 2 * nc - reCol, X.pcoeff * (dLsidedp + dRsidedp));
+    if (M.nw) {   // wells: PIw in the P column (matrix :36029, fracture :18474)
+        const WellQ<double> wq = well_terms<double>(M, kc, c, ix, iy, iz, X.S->P[kc][c], X.S->P);
+        out.add(jacCol + 2 * nc - reCol, X.pcoeff * wq.PIw);
+    }
     const double dPw = (f.P[3 * c] - m.P[3 * c]) + (m.pc[2 * c] - f.pc[2 * c]);
     const double cS = SI_C * WAT_M_RO;
     double dRsidedSw, dRsideOdSw;
@@ -1030,6 +1053,7 @@ int64_t or_jacobian_ref_mt(const or_fluid* fl, const or_fluid_ext* fe, const or_
     ref_cells(M, S, 0, R[0], nthreads);
     ref_cells(M, S, 1, R[1], nthreads);
     Ctx X(M);
+    X.S = &S;
     X.R[0] = &R[0];
     X.R[1] = &R[1];
     X.n = n;

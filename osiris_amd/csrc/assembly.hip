@@ -2501,6 +2501,9 @@ int rj_jacobian_nc(osr_asm* a, const osr_state_dev* s, const int64_t* rowptr, in
                        a->dprops);
     hipLaunchKernelGGL(osr::k_rj_rows<NC>, dim3((unsigned)((nb * MAXC + 127) / 128)), dim3(128), 0, st, A,
                        (const double*)a->dprops, rowptr, cols, vals);
+    if (a->nw > 0 && a->nwb > 0)
+        hipLaunchKernelGGL(osr::k_well_refjac<NC>, dim3((a->nwb + 63) / 64), dim3(64), 0, st, A,
+                           (const int*)a->d_wblk, a->nwb, rowptr, (const int32_t*)cols, vals);
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
 }
 
@@ -3062,7 +3065,6 @@ int osr_jacobian_ref_device(osr_asm* a, const osr_state_dev* s, const int64_t* r
     int rc = check_state(a, s);
     if (rc) return rc;
     if (!rowptr || !cols || !vals) return OSR_E_ARG;
-    if (a->nw > 0) return OSR_E_ARG;   // the reference rules of the well columns are not restated
 #define CALL(NC) rj_jacobian_nc<NC>(a, s, rowptr, cols, vals, (hipStream_t)stream)
     OSR_NC_SWITCH(a->nc, CALL)
 #undef CALL

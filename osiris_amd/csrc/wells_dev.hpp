@@ -274,16 +274,23 @@ __device__ __forceinline__ void well_one(const AsmArgs& A, int i, int k, int c, 
     }
 }
 
-// Rates of the wells completed in block (k, c); returns the injector whose
-// stream the component rows take, or -1.
+// Rates and productivity indices of the wells completed in block (k, c).
 template <int NC, class T>
-__device__ __forceinline__ int well_rates(const AsmArgs& A, int k, int c, const Geo& G, const PropsT<NC, T>& p,
-                                          T& Qw, T& Qo, T& Qg) {
+__device__ __forceinline__ WellAcc<T> well_acc(const AsmArgs& A, int k, int c, const Geo& G, const PropsT<NC, T>& p) {
     WellAcc<T> W{T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), -1};
     for (int i = 0; i < A.nw; i++) {
         const int* wi = A.wi + 5 * i;
         if (G.ix == wi[1] && G.iy == wi[2] && G.iz >= wi[3] && G.iz <= wi[4]) well_one<NC, T>(A, i, k, c, G, p, W);
     }
+    return W;
+}
+
+// Rates of the wells completed in block (k, c); returns the injector whose
+// stream the component rows take, or -1.
+template <int NC, class T>
+__device__ __forceinline__ int well_rates(const AsmArgs& A, int k, int c, const Geo& G, const PropsT<NC, T>& p,
+                                          T& Qw, T& Qo, T& Qg) {
+    const WellAcc<T> W = well_acc<NC, T>(A, k, c, G, p);
     Qw = W.Qw;
     Qo = W.Qo;
     Qg = W.Qg;
@@ -497,6 +504,56 @@ __global__ void k_well_jac(AsmArgs A, const int* wblk, int nwb, const double* dp
 #pragma unroll
     for (int i = 0; i < NC; i++) vals[rowptr[rb + i] + s_mass + j] += R.bal[i].d[0];
     vals[rowptr[rb + NC] + seg_start(A, k, G, c, 1, blk, NC) + j] += R.bal[NC].d[0];
+}
+
+// reference-mode Jacobian: the reference's well entries (IteratorJacobian,
+// matrix :32543, :32808-32813, :35425, :36029; fracture :15033, :17868,
+// :18474) added into the block's own columns of its balance and water rows:
+// PCoeff (x_i PIo + y_i PIg) in the P column of component row i (y_i the
+// injection stream's in an injector's block; x_i the block's own in every
+// case, as there), PCoeff (PIo + PIg) in the total row's, PCoeff PIw in the
+// water row's, and for a producer -Qo, -Qg in the x_i, y_i columns of
+// component row i.  One lane per completed block; the columns are found in
+// the row's sorted column list.
+__device__ __forceinline__ void csr_add(const int64_t* rowptr, const int32_t* cols, double* vals, int64_t row,
+                                        int64_t col, double v) {
+    int64_t lo = rowptr[row], hi = rowptr[row + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cols[mid] < col) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < rowptr[row + 1] && cols[lo] == col) vals[lo] += v;
+}
+
+template <int NC>
+__global__ void k_well_refjac(AsmArgs A, const int* wblk, int nwb, const int64_t* rowptr, const int32_t* cols,
+                              double* vals) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nwb) return;
+    const int blk = wblk[t], n = A.n, k = blk / n, c = blk % n;
+    if (A.phi[k * n + c] < 1e-10) return;
+    const int lab = A.lab[k][c];
+    PropsT<NC, double> p;
+    load_props<NC>(A, k, c, p);
+    const Geo G = geom(A, c);
+    const WellAcc<double> W = well_acc<NC, double>(A, k, c, G, p);
+    const int64_t b0 = A.boff[blk];
+    const int r0 = lab == 0 ? NC : 0, reCol = lab == 0 ? 0 : NC;
+    const int64_t pcol = b0 + 2 * NC - reCol;
+    const double* ic = W.inj >= 0 ? A.winj + (int64_t)(2 * NC + 6) * W.inj : nullptr;
+    for (int i = 0; i < NC - 1; i++) {
+        const double xi = A.x[k][(int64_t)i * n + c];
+        const double gc = ic ? ic[NC + i] : A.y[k][(int64_t)i * n + c];
+        const int64_t row = b0 + r0 + i;
+        csr_add(rowptr, cols, vals, row, pcol, A.pcoeff * (xi * W.PIo + gc * W.PIg));
+        if (!ic) {
+            if (lab != -1) csr_add(rowptr, cols, vals, row, b0 + i, -W.Qo);
+            if (lab != 1) csr_add(rowptr, cols, vals, row, b0 + NC + i - reCol, -W.Qg);
+        }
+    }
+    csr_add(rowptr, cols, vals, b0 + r0 + NC - 1, pcol, A.pcoeff * (W.PIo + W.PIg));
+    csr_add(rowptr, cols, vals, b0 + r0 + NC, pcol, A.pcoeff * W.PIw);
 }
 
 // reduced system (direct): the own column block of the completed block's

@@ -178,10 +178,6 @@ def test_wells_reset_in_place_and_removed(gpu, oracle):
     dp = plain.device_state(st)
     plain.layout(dp)
     assert np.array_equal(F0, plain.residual(dp).cpu().numpy())
-    # the reference-rule Jacobian has no restated well columns: refused
-    asm.set_wells(w2)
-    with pytest.raises(RuntimeError):
-        asm.jacobian(d, reference=True)
 
 
 def test_example_deck_wells_on_the_device(gpu, oracle):
@@ -213,3 +209,24 @@ def test_example_deck_wells_on_the_device(gpu, oracle):
         np.maximum.at(rowmax, row, np.abs(valso))
         v = vals.cpu().numpy()
         assert np.max(np.abs(v - valso) / np.maximum(np.abs(valso), 1e-8 * rowmax[row] + 1e-300)) <= RTOL
+
+
+def test_reference_jacobian_with_wells(gpu, oracle):
+    """The reference-mode Jacobian (hand rules, its own pattern) with wells
+    against the oracle's restatement, at the reference-mode bar: pattern
+    identical, values within 1e-8 relative floored at 1e-8 of the row."""
+    from tests.test_gpu_refjac import _compare
+    for name, (fl, grid, st) in _cases(oracle):
+        if name == "c0_all_types_raw":
+            continue
+        asm = gpu.Assembler(gpu.Context(fl), fl, grid)
+        d = asm.device_state(st)
+        asm.layout(d)
+        rp, cols, vals = asm.jacobian(d, reference=True)
+        torch.cuda.synchronize()
+        g = (rp.cpu().numpy(), cols.cpu().numpy(), vals.cpu().numpy())
+        r = oracle.jacobian_ref(fl, grid, st)
+        _compare(name, g, r)
+        # the wells change the reference-mode Jacobian
+        r0 = oracle.jacobian_ref(fl, _no_wells(grid), st)
+        assert np.count_nonzero(r[2] != r0[2]) > 0, name

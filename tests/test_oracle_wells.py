@@ -303,3 +303,79 @@ def _replace(st, **kw):
     for k, v in kw.items():
         setattr(s, k, v)
     return s
+
+
+# ------------------------------------------- reference-mode Jacobian
+def _dense(rpc):
+    rp, cols, vals = rpc
+    N = len(rp) - 1
+    return sp.csr_matrix((vals, cols, rp), shape=(N, N))
+
+
+@pytest.mark.parametrize("wtype", (0, 1, 2, 3, 4, 7, 8))
+def test_reference_jacobian_well_columns(oracle, wtype):
+    """The reference's well entries (IteratorJacobian, matrix :32543,
+    :32808-32813, :35425, :36029; fracture :15033, :17868, :18474): the
+    productivity indices PI = Q / (Pwf - P), read off the residual's rates,
+    in the block's P column (times PCoeff; component row i: x_i PIo + y_i
+    PIg with the injection stream's y_i in an injector's block, total PIo +
+    PIg, water PIw) and, for a producer, -Qo, -Qg in the block's x_i, y_i
+    columns; nothing elsewhere."""
+    fl, grid, st = c0_case(oracle, nx=4, ny=3, nz=2, seed=1, sw_m=0.45, sw_f=0.3)
+    nc, n = fl.nc, grid.n
+    ix, iy = 1, 1
+    s0 = column_state(st, grid, ix, iy, Po=(10e6, 10.2e6), label=0, x=[0.3, 0.3, 0.4],
+                      y=[0.8, 0.15, 0.05], S=([0.3, 0.6, 0.1], [0.3, 0.5, 0.2]))
+    z = np.array([0.6, 0.1, 0.3])
+    rows = {0: wrow(nc, 0, ix, iy, 0, 1, 8e6, 0.0), 1: wrow(nc, 1, ix, iy, 0, 1, 8e6, 0.0),
+            2: wrow(nc, 2, ix, iy, 0, 1, 20e6, 0.0), 3: wrow(nc, 3, ix, iy, 0, 1, 13e6, 0.0, z=z),
+            4: wrow(nc, 4, ix, iy, 0, 1, 3.0, 0.0), 7: wrow(nc, 7, ix, iy, 0, 1, 20e6, 0.0),
+            8: wrow(nc, 8, ix, iy, 0, 1, 40.0, 13e6, z=z)}
+    cond = [1] if wtype in (7, 8) else [0]
+    g = with_wells(grid, [rows[wtype]], cond=cond)
+    bs, off = oracle.layout(fl, g, s0)
+    dF = oracle.residual(fl, g, s0) - oracle.residual(fl, grid, s0)
+    dJ = (_dense(oracle.jacobian_ref(fl, g, s0)) - _dense(oracle.jacobian_ref(fl, grid, s0))).tocoo()
+    pr = oracle.props(fl, g, s0)
+    _, inj = oracle.well_data(fl, g, s0)
+    pwf = rows[wtype][1][0] if wtype != 8 else rows[wtype][1][1]
+    expect = {}
+    for k in range(2):
+        for iz in (0, 1):
+            c = cell(grid, ix, iy, iz)
+            blk = k * n + c
+            r = bal_rows(fl, s0, off, blk, n)
+            x = (s0.x_m if k == 0 else s0.x_f)[:, c]
+            y = (s0.y_m if k == 0 else s0.y_f)[:, c]
+            injector = wtype in (3, 8)
+            xo, yg = (inj[0, :nc], inj[0, nc:2 * nc]) if injector else (x, y)
+            # rates from the residual: [comp rows..., total] = -A [Qo, Qg]
+            A = np.vstack([np.column_stack([xo[:nc - 1], yg[:nc - 1]]), [1.0, 1.0]])
+            q, *_ = np.linalg.lstsq(A, -dF[r[:nc]], rcond=None)
+            Qo, Qg, Qw = q[0], q[1], -dF[r[nc]]
+            Pw, Po, Pg = pr[k, c, 1], pr[k, c, 2], pr[k, c, 3]
+            pressure_rates = wtype in (0, 1, 2, 3, 7) or (wtype == 8 and cond[0])
+            if pressure_rates:
+                PIo = Qo / (pwf - (Pg if injector else Po))
+                PIg = Qg / (pwf - Pg)
+                PIw = Qw / (pwf - Pw)
+            else:
+                PIo = PIg = PIw = 0.0
+            pcol = off[blk] + 2 * nc            # two-phase block: [x, y, P, ...]
+            for i in range(nc - 1):
+                expect[(r[i], pcol)] = s0.pcoeff * (x[i] * PIo + yg[i] * PIg)
+                if not injector:
+                    expect[(r[i], off[blk] + i)] = -Qo
+                    expect[(r[i], off[blk] + nc + i)] = -Qg
+            expect[(r[nc - 1], pcol)] = s0.pcoeff * (PIo + PIg)
+            expect[(r[nc], pcol)] = s0.pcoeff * PIw
+    got = {}
+    for a, b, v in zip(dJ.row, dJ.col, dJ.data):
+        if v != 0:
+            got[(a, b)] = got.get((a, b), 0.0) + v
+    scale = max(abs(v) for v in expect.values())
+    assert sum(abs(v) > 1e-9 * scale for v in expect.values()) >= 4, "the well sets entries"
+    for key, v in expect.items():
+        assert got.get(key, 0.0) == pytest.approx(v, rel=1e-7, abs=1e-12 * scale), (wtype, key)
+    extra = {kk: v for kk, v in got.items() if kk not in expect and abs(v) > 1e-12 * scale}
+    assert not extra, (wtype, extra)
