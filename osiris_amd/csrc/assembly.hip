@@ -31,6 +31,7 @@
 #include "dual_dev.hpp"
 #include "grp_dev.hpp"
 #include "osr_internal.hpp"
+#include "phase_dev.hpp"
 
 namespace osr {
 
@@ -2308,6 +2309,49 @@ __global__ void k_block_sizes(const int8_t* lm, const int8_t* lf, int n, int nc,
     bs[blk] = L == 0 ? 2 * nc + 4 : nc + 3;
 }
 
+// UpdateAll(X, true)'s last two stages in one pass per block (PetscAUX.cpp
+// :176-507): the PhaseCtrl / fPhaseCtrl rule of the block's cell (its
+// saturation pressure and incipient composition already evaluated; Pg0 the
+// gas pressure of the previous sweep, which the reference's PhaseCtrl reads
+// before the sweep refreshes it), then UpdateDependencies' property sweep
+// of the new state with the previous sweep's densities (k_props), the
+// block size of the new label (PJac) and, for a matrix block, the densities
+// the next sweep's CalcIFT reads.  The state arrays of A alias the writable
+// ones passed here; each lane reads back only what it wrote.
+template <int NC>
+__global__ __launch_bounds__(256) void k_apply_sweep(AsmArgs A, int8_t* lab_m, int8_t* lab_f, double* x_m,
+                                                     double* y_m, double* x_f, double* y_f, double* S_m,
+                                                     double* S_f, const double* psat, const double* xm,
+                                                     int64_t* bs, double* rho_next) {
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= 2 * A.n) return;
+    const int n = A.n, k = blk / n, c = blk % n;
+    const int64_t s = 2 * (int64_t)n;
+    double* q = A.props + blk;
+    phase_apply_cell<NC>(n, c, k == 0 ? lab_m : lab_f, A.Po[k][c], q[3 * s], k == 0 ? x_m : x_f,
+                         k == 0 ? y_m : y_f, k == 0 ? S_m : S_f, psat[blk], xm + (int64_t)k * NC * n);
+    VarsT<NC, double> v;
+    load_vars<NC>(A, k, c, v);
+    double ro = 0, rg = 0;
+    rho_prev_of<NC>(A, c, ro, rg);
+    PropsT<NC, double> p;
+    propsT<NC, double>(A, k, c, v.lab, v.Po, v.x, v.y, v.S, ro, rg, p);
+    q[0 * s] = p.porpor; q[1 * s] = p.Pw; q[2 * s] = p.Po; q[3 * s] = p.Pg; q[4 * s] = p.bift;
+    q[5 * s] = p.ro[0]; q[6 * s] = p.ro[1]; q[7 * s] = p.mu[0]; q[8 * s] = p.mu[1];
+    q[9 * s] = p.gam[0]; q[10 * s] = p.gam[1];
+    q[11 * s] = p.kr[0]; q[12 * s] = p.kr[1]; q[13 * s] = p.kr[2]; q[14 * s] = p.pcw; q[15 * s] = p.pcg;
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        q[(16 + i) * s] = p.D[0][i];
+        q[(16 + NC + i) * s] = p.D[1][i];
+    }
+    bs[blk] = v.lab == 0 ? 2 * NC + 4 : NC + 3;
+    if (k == 0 && rho_next) {
+        rho_next[c] = p.ro[0];
+        rho_next[n + c] = p.ro[1];
+    }
+}
+
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
 }  // namespace osr
@@ -3314,6 +3358,44 @@ int asm_sweep(osr_asm* a, const osr_state_dev* s, double* rho_out, hipStream_t s
 #undef CALL
     if (rc) return rc;
     return rho_out ? asm_take_rho(a, rho_out, st) : OSR_OK;
+}
+
+// PhaseCtrl's rule, the property sweep and the layout of the new state in
+// one kernel plus the block-offset scan (k_apply_sweep); *nrows (host)
+// receives the new MatrixSize after one synchronisation of the stream.
+int asm_apply_sweep(osr_asm* a, const osr_state_mut_dev* m, const double* rho_prev, const double* psat,
+                    const double* xm, double* rho_next, int64_t* nrows, hipStream_t st) {
+    osr_state_dev s{};
+    s.label_m = m->label_m; s.label_f = m->label_f; s.Po_m = m->Po_m; s.Po_f = m->Po_f;
+    s.x_m = m->x_m; s.y_m = m->y_m; s.x_f = m->x_f; s.y_f = m->y_f; s.S_m = m->S_m; s.S_f = m->S_f;
+    s.rho_prev_m = rho_prev;
+    s.pcoeff = m->pcoeff;
+    if (!rho_prev || !psat || !xm || !nrows) return OSR_E_ARG;
+    const int rc0 = check_state(a, &s);
+    if (rc0) return rc0;
+    osr::AsmArgs A = make_args(a, &s);
+    const int nb = 2 * a->n;
+    int8_t* lm = (int8_t*)m->label_m;
+    int8_t* lf = (int8_t*)m->label_f;
+    switch (a->nc) {
+#define APPLY_(NC)                                                                                            \
+    case NC:                                                                                                  \
+        hipLaunchKernelGGL(osr::k_apply_sweep<NC>, dim3((nb + 255) / 256), dim3(256), 0, st, A, lm, lf, m->x_m, \
+                           m->y_m, m->x_f, m->y_f, m->S_m, m->S_f, psat, xm, a->bs, rho_next);                 \
+        break;
+        OSR_NC_LIST(APPLY_)
+#undef APPLY_
+        default: return OSR_E_NC;
+    }
+    OSR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(osr::k_set_i64, dim3(1), dim3(1), 0, st, a->bs + nb, (int64_t)0);
+    OSR_HIP_RET(exclusive_scan(a, a->bs, a->boff, nb + 1, st));
+    int64_t total = 0;
+    OSR_HIP(hipMemcpyAsync(&total, a->boff + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    OSR_HIP(hipStreamSynchronize(st));
+    a->nrows = total;
+    *nrows = total;
+    return OSR_OK;
 }
 
 // The matrix oil/gas densities of the table as the last sweep left it.

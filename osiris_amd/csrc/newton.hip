@@ -114,7 +114,7 @@ struct osr_newton {
     int32_t* bcols = nullptr;
     double* bvals = nullptr;
     int64_t nnzb_cap = 0;
-    double *rho0 = nullptr, *psat = nullptr, *part = nullptr, *sc = nullptr;
+    double *rho0 = nullptr, *psat = nullptr, *xm = nullptr, *part = nullptr, *sc = nullptr;
     double* host_sc = nullptr;  // pinned [4]
     int64_t last_nrows = 0;
 };
@@ -123,7 +123,8 @@ namespace {
 
 void release(osr_newton* w) {
     if (w->ksp) osr_ksp_destroy(w->ksp);
-    double* d[] = {w->X, w->F, w->du, w->brhs, w->dp, w->S, w->t, w->bvals, w->rho0, w->psat, w->part, w->sc};
+    double* d[] = {w->X, w->F, w->du, w->brhs, w->dp, w->S, w->t, w->bvals, w->rho0, w->psat, w->xm, w->part,
+                   w->sc};
     for (double* p : d)
         if (p) hipFree(p);
     if (w->browptr) hipFree(w->browptr);
@@ -179,6 +180,7 @@ int osr_newton_create(osr_asm* a, osr_newton** out) {
               hipMalloc((void**)&w->browptr, (nb + 1) * sizeof(int64_t)) == hipSuccess &&
               hipMalloc((void**)&w->rho0, nb * sizeof(double)) == hipSuccess &&
               hipMalloc((void**)&w->psat, nb * sizeof(double)) == hipSuccess &&
+              hipMalloc((void**)&w->xm, nb * nc * sizeof(double)) == hipSuccess &&
               hipMalloc((void**)&w->part, 3 * osr::NRM_BLOCKS * sizeof(double)) == hipSuccess &&
               hipMalloc((void**)&w->sc, 4 * sizeof(double)) == hipSuccess &&
               hipHostMalloc((void**)&w->host_sc, 4 * sizeof(double), hipHostMallocDefault) == hipSuccess;
@@ -278,25 +280,27 @@ int osr_newton_step_device(osr_newton* w, osr_state_rw_dev* s, const osr_newton_
     OSR_HIP_RET(osr_unpack_device(a, &sm, w->X, stream));
 
     // UpdateDependencies(true): PhaseCtrl, fPhaseCtrl (gas pressures of X0's
-    // sweep, still in the property table), CalcPCoeff, PJac, then the sweep
+    // sweep, still in the property table), CalcPCoeff, PJac, then the sweep.
+    // The saturation pressures the phase test needs first (matrix, then
+    // fracture), the largest pressure (PCoeff), then one kernel per block
+    // applies the phase rule and runs the new state's property sweep and
+    // block size, and the offsets are scanned; one synchronisation reads
+    // PCoeff and MatrixSize.
     const double* props = nullptr;
     OSR_HIP_RET(osr_asm_props_table(a, &props));
     const double* Pg = props + 3 * 2 * n;
-    OSR_HIP_RET(osr::phase_ctrl(w->ctx, n, nullptr, w->T, s->label_m, s->Po_m, Pg, s->x_m, s->y_m, s->S_m,
-                                w->psat, st));
-    OSR_HIP_RET(osr::phase_ctrl(w->ctx, n, nullptr, w->T, s->label_f, s->Po_f, Pg + n, s->x_f, s->y_f, s->S_f,
-                                w->psat + n, st));
+    OSR_HIP_RET(osr::phase_psat(w->ctx, n, nullptr, w->T, s->label_m, s->Po_m, Pg, s->x_m, s->y_m, w->psat,
+                                w->xm, st));
+    OSR_HIP_RET(osr::phase_psat(w->ctx, n, nullptr, w->T, s->label_f, s->Po_f, Pg + n, s->x_f, s->y_f,
+                                w->psat + n, w->xm + (size_t)w->nc * n, st));
     hipLaunchKernelGGL(osr::k_step_pmax, dim3(osr::NRM_BLOCKS), dim3(256), 0, st, n, s->Po_m, s->Po_f, w->part);
     hipLaunchKernelGGL(osr::k_step_final, dim3(1), dim3(256), 0, st, w->part, w->sc, 1);
     OSR_HIP(hipGetLastError());
     OSR_HIP(hipMemcpyAsync(w->host_sc + 2, w->sc + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-    OSR_HIP(hipStreamSynchronize(st));
+    int64_t nrows_next = 0;
+    OSR_HIP_RET(osr::asm_apply_sweep(a, &sm, w->rho0, w->psat, w->xm, rho_next, &nrows_next, st));
     if (!(w->host_sc[2] > 0)) return OSR_E_ARG;   // no positive pressure left: no PCoeff to scale by
     s->pcoeff = w->host_sc[2];
-    const osr_state_dev sn = const_view(s, w->rho0);
-    int64_t nrows_next = 0;
-    OSR_HIP_RET(osr_asm_layout(a, &sn, nullptr, &nrows_next, stream));
-    OSR_HIP_RET(osr::asm_sweep(a, &sn, rho_next, st));
     r->nrows_next = nrows_next;
     r->accepted = 1;
     return OSR_OK;

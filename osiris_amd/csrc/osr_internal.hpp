@@ -94,4 +94,8 @@ void asm_info(const osr_asm* a, osr_ctx** ctx, int64_t* n, int* nc, double* T);
 void asm_dims(const osr_asm* a, int* nx, int* ny, int* nz);
 int asm_sweep(osr_asm* a, const osr_state_dev* s, double* rho_out, hipStream_t st);
 int asm_take_rho(osr_asm* a, double* rho_out, hipStream_t st);
+int asm_apply_sweep(osr_asm* a, const osr_state_mut_dev* m, const double* rho_prev, const double* psat,
+                    const double* xm, double* rho_next, int64_t* nrows, hipStream_t st);
+int phase_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const int8_t* label, const double* Po,
+               const double* Pg, const double* x, const double* y, double* psat, double* xm, hipStream_t st);
 }  // namespace osr

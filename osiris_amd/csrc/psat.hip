@@ -13,6 +13,7 @@
 #include <algorithm>
 
 #include "osr_internal.hpp"
+#include "phase_dev.hpp"
 
 namespace osr {
 
@@ -424,10 +425,6 @@ int launch_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const doubl
 // k_phase_kind picks the saturation test per cell (dew at the gas pressure
 // for gas cells, bubble at the oil pressure for liquid cells), k_psat runs
 // it, k_phase_apply applies the label/saturation/composition rules.
-constexpr double PHASE_ENV = 0.95;   // PAHSE_ENV_BOUNDARY, Functions.h:11
-constexpr double EPS_SAT = 1e-3;     // Functions.h:226
-constexpr double EPS_SAT_FRAC = 0;   // Functions.h:12
-
 __global__ void k_phase_kind(int64_t n, const int8_t* label, const double* Po, const double* Pg,
                              int8_t* kind, double* Pin) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -442,42 +439,7 @@ __global__ void k_phase_apply(int64_t n, int8_t* label, const double* Po, const 
                               double* y, double* S, const double* psat, const double* xm) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    int L = label[c];
-    double Sw = S[c], So = S[n + c], Sg = S[2 * n + c];
-    const double ps = psat[c];
-    bool done = false;
-    if (L == -1 && Pg[c] <= PHASE_ENV * ps) {
-        So = EPS_SAT;
-        Sg -= EPS_SAT;
-        L = 0;
-#pragma unroll
-        for (int i = 0; i < NC; i++) x[(int64_t)i * n + c] = xm[(int64_t)i * n + c];
-        done = true;
-    } else if (L == 1 && Po[c] <= PHASE_ENV * ps) {
-        Sg = EPS_SAT;
-        So -= EPS_SAT;
-        L = 0;
-#pragma unroll
-        for (int i = 0; i < NC; i++) y[(int64_t)i * n + c] = xm[(int64_t)i * n + c];
-        done = true;
-    }
-    if (!done) {
-        if (So <= EPS_SAT * EPS_SAT_FRAC) {
-            So = 0;
-            Sg = 1 - Sw;
-            L = -1;
-        }
-        if (Sg <= EPS_SAT * EPS_SAT_FRAC) {
-            Sg = 0;
-            So = 1 - Sw;
-            L = 1;
-        }
-    }
-    const double inv = 1.0 / (Sw + So + Sg);
-    S[c] = Sw * inv;
-    S[n + c] = So * inv;
-    S[2 * n + c] = Sg * inv;
-    label[c] = (int8_t)L;
+    phase_apply_cell<NC>(n, c, label, Po[c], Pg[c], x, y, S, psat[c], xm);
 }
 
 int phase_ctrl(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* label, const double* Po,
@@ -513,6 +475,31 @@ int phase_ctrl(osr_ctx* ctx, int64_t n, const double* T, double T0, int8_t* labe
     }
     if (psat_out) OSR_HIP(hipMemcpyAsync(psat_out, ps, n * sizeof(double), hipMemcpyDeviceToDevice, st));
     return hipGetLastError() == hipSuccess ? OSR_OK : OSR_E_HIP;
+}
+
+// The saturation pressures PhaseCtrl tests (kind and Bubl/Dew_Succ) without
+// applying them: psat [n] and the incipient compositions xm [nc*n] into the
+// caller's arrays, for a kernel that applies the rule itself.
+int phase_psat(osr_ctx* ctx, int64_t n, const double* T, double T0, const int8_t* label, const double* Po,
+               const double* Pg, const double* x, const double* y, double* psat, double* xm, hipStream_t st) {
+    if (!ctx || n < 0) return OSR_E_ARG;
+    if (n == 0) return OSR_OK;
+    if (!label || !Po || !Pg || !x || !y || !psat || !xm) return OSR_E_ARG;
+    if (!T && !(T0 > 0)) return OSR_E_ARG;
+    const size_t need = n * (sizeof(int8_t) + sizeof(double)) + 1024;
+    if (need > ctx->queue_bytes) {
+        if (ctx->ws_frozen) return OSR_E_SIZE;
+        if (ctx->d_queue) hipFree(ctx->d_queue);
+        ctx->d_queue = nullptr;
+        ctx->queue_bytes = 0;
+        OSR_HIP(hipMalloc(&ctx->d_queue, need));
+        ctx->queue_bytes = need;
+    }
+    double* Pin = (double*)ctx->d_queue;
+    int8_t* kind = (int8_t*)(Pin + n);
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_phase_kind, dim3(g), dim3(256), 0, st, n, label, Po, Pg, kind, Pin);
+    return launch_psat(ctx, n, T, T0, Pin, kind, x, y, psat, xm, st);
 }
 
 }  // namespace osr

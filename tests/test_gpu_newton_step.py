@@ -70,7 +70,9 @@ def test_accepted_step_matches_the_oracle_chain(gpu, oracle, dt, tolmain):
     asm = gpu.Assembler(ctx, fl, grid)
     d = asm.device_state(s)
     asm.layout(d)
-    pg0 = asm.props(d)[3].clone()              # the gas pressures of X0's sweep on the GPU
+    p0 = asm.props(d)
+    pg0 = p0[3].clone()                        # the gas pressures of X0's sweep on the GPU
+    rho0 = torch.cat([p0[5, :grid.n], p0[6, :grid.n]]).clone()   # its matrix densities
     nw = gpu.NewtonStep(asm)
     res = nw.step(d, tolmain=tolmain, rtol=1e-12, maxit=2000)
     torch.cuda.synchronize()
@@ -150,6 +152,12 @@ def test_accepted_step_matches_the_oracle_chain(gpu, oracle, dt, tolmain):
     rho = d["rho_prev_m"].cpu().numpy().reshape(2, -1)
     np.testing.assert_allclose(rho[0], pr1[0, :, 5], rtol=1e-12)
     np.testing.assert_allclose(rho[1], pr1[0, :, 6], rtol=1e-12)
+    # the step's fused PhaseCtrl-and-sweep kernel gave bitwise the property
+    # sweep of the new state run on its own (with X0's densities)
+    d2 = dict(d)
+    d2["rho_prev_m"] = rho0
+    p1 = asm.props(d2)
+    assert torch.equal(d["rho_prev_m"].reshape(-1), torch.cat([p1[5, :grid.n], p1[6, :grid.n]]))
 
     # the next step starts from that state: its residual is the oracle's
     got.rho_prev_m = rho
