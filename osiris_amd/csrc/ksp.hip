@@ -10,8 +10,12 @@
 // fused vector updates that read their scalars from device memory, and
 // dot products in two fixed-shape stages (a fixed grid of block partials,
 // then one block) so every reduction has one summation order and the solve
-// is bitwise reproducible.  The host loop synchronises once per iteration to
-// read the residual norm and the breakdown flag.
+// is bitwise reproducible.  The stopping test runs on the device after each
+// iteration (k_ksp_check: residual norm, tolerances, divergence, breakdown,
+// iteration cap) and sets a done flag that every iteration kernel reads
+// first, so the host launches iterations in growing chunks (1, 2, 4, ...,
+// 16) and synchronises once per chunk; the iterations after the stop are
+// empty launches and leave every vector as the stopping iteration left it.
 #include <algorithm>
 #include <cmath>
 #include <new>
@@ -25,7 +29,15 @@ constexpr int KG = 16;          // lanes per block row in the inverse kernel
 constexpr int DOT_BLOCKS = 1024;
 
 // scalar slots on the device
-enum { SC_RHO = 0, SC_ALPHA, SC_OMEGA, SC_BETA, SC_RN2, SC_FLAG, SC_NSING, SC_COUNT = 8 };
+enum {
+    SC_RHO = 0, SC_ALPHA, SC_OMEGA, SC_BETA, SC_RN2, SC_FLAG, SC_NSING,
+    // the device stopping test: state, then its parameters
+    SC_DONE, SC_ITS, SC_RN, SC_REASON, SC_TOL, SC_ATOL, SC_DTOLR0, SC_MAXIT,
+    SC_COUNT = 16
+};
+// true once the device stopping test has ended the solve (sc may be null
+// outside the iteration loop)
+__device__ __forceinline__ bool ksp_done(const double* sc) { return sc && sc[SC_DONE] != 0.0; }
 
 template <int BS>
 __global__ __launch_bounds__(256) void k_bj_inv(int64_t nbr, const int64_t* browptr, const int32_t* bcols,
@@ -96,9 +108,10 @@ __global__ __launch_bounds__(256) void k_bj_inv(int64_t nbr, const int64_t* brow
 // y = A x (BSR), one lane per scalar row
 template <int BS>
 __global__ __launch_bounds__(256) void k_bsr_spmv(int64_t nbr, const int64_t* browptr, const int32_t* bcols,
-                                                  const double* bvals, const double* x, double* y) {
+                                                  const double* bvals, const double* x, double* y,
+                                                  const double* sc) {
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= nbr * BS) return;
+    if (item >= nbr * BS || ksp_done(sc)) return;
     const int64_t br = item / BS;
     const int r = (int)(item % BS);
     double acc = 0.0;
@@ -113,9 +126,10 @@ __global__ __launch_bounds__(256) void k_bsr_spmv(int64_t nbr, const int64_t* br
 
 // z = M^-1 y with M the block diagonal
 template <int BS>
-__global__ __launch_bounds__(256) void k_bj_apply(int64_t nbr, const double* minv, const double* y, double* z) {
+__global__ __launch_bounds__(256) void k_bj_apply(int64_t nbr, const double* minv, const double* y, double* z,
+                                                  const double* sc) {
     const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= nbr * BS) return;
+    if (item >= nbr * BS || ksp_done(sc)) return;
     const int64_t br = item / BS;
     const int r = (int)(item % BS);
     const double* row = minv + br * BS * BS + r * BS;
@@ -128,7 +142,8 @@ __global__ __launch_bounds__(256) void k_bj_apply(int64_t nbr, const double* min
 
 // partial sums of up to two dot products (a.b, c.d) over a fixed grid
 __global__ __launch_bounds__(256) void k_dot2(int64_t n, const double* a, const double* b, const double* c,
-                                              const double* d, double* part) {
+                                              const double* d, double* part, const double* sc) {
+    if (ksp_done(sc)) return;
     __shared__ double s0[256], s1[256];
     double v0 = 0.0, v1 = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
@@ -155,6 +170,7 @@ enum { OP_RHO = 0, OP_ALPHA, OP_OMEGA, OP_RNORM, OP_INIT };
 
 // final reduction of the partials and the scalar recurrences of BiCGStab
 __global__ __launch_bounds__(256) void k_dot_final(const double* part, double* sc, int op) {
+    if (sc[SC_DONE] != 0.0) return;
     __shared__ double s0[256], s1[256];
     double v0 = 0.0, v1 = 0.0;
     for (int i = threadIdx.x; i < DOT_BLOCKS; i += 256) {
@@ -218,14 +234,14 @@ __global__ void k_vec_init(int64_t n, const double* b, const double* ax, double*
 // p = r + beta (p - omega v)
 __global__ void k_vec_p(int64_t n, const double* sc, const double* r, const double* v, double* p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || ksp_done(sc)) return;
     p[i] = r[i] + sc[SC_BETA] * (p[i] - sc[SC_OMEGA] * v[i]);
 }
 
 // s = r - alpha v
 __global__ void k_vec_s(int64_t n, const double* sc, const double* r, const double* v, double* s) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || ksp_done(sc)) return;
     s[i] = r[i] - sc[SC_ALPHA] * v[i];
 }
 
@@ -233,7 +249,7 @@ __global__ void k_vec_s(int64_t n, const double* sc, const double* r, const doub
 __global__ void k_vec_xr(int64_t n, const double* sc, const double* ph, const double* sh, const double* s,
                          const double* t, double* x, double* r) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || ksp_done(sc)) return;
     const double om = sc[SC_OMEGA];
     x[i] += sc[SC_ALPHA] * ph[i] + om * sh[i];
     r[i] = s[i] - om * t[i];
@@ -246,6 +262,38 @@ __global__ void k_vec_zero(int64_t n, double* x) {
 
 __global__ void k_sc_zero(double* sc) {
     if (threadIdx.x < SC_COUNT) sc[threadIdx.x] = 0.0;
+}
+
+// the stopping test's parameters; the solve is not done, no iteration run
+__global__ void k_ksp_params(double* sc, double tol, double atol, double dtolr0, double maxit) {
+    sc[SC_DONE] = 0.0;
+    sc[SC_ITS] = 0.0;
+    sc[SC_REASON] = 0.0;
+    sc[SC_TOL] = tol;
+    sc[SC_ATOL] = atol;
+    sc[SC_DTOLR0] = dtolr0;
+    sc[SC_MAXIT] = maxit;
+}
+
+// after an iteration's |r|^2: the host loop's tests in its order -- a
+// non-finite norm, the tolerance (absolute if below atol), divergence
+// against dtol |r0|, a breakdown flag of the iteration, the iteration cap
+__global__ void k_ksp_check(double* sc) {
+    if (sc[SC_DONE] != 0.0) return;
+    const double its = sc[SC_ITS] + 1.0;
+    sc[SC_ITS] = its;
+    const double rn = sqrt(sc[SC_RN2]);
+    sc[SC_RN] = rn;
+    double reason = 0.0;
+    if (!isfinite(rn)) reason = OSR_KSP_DIVERGED_NANORINF;
+    else if (rn <= sc[SC_TOL]) reason = rn <= sc[SC_ATOL] ? OSR_KSP_CONVERGED_ATOL : OSR_KSP_CONVERGED_RTOL;
+    else if (rn >= sc[SC_DTOLR0]) reason = OSR_KSP_DIVERGED_DTOL;
+    else if (sc[SC_FLAG] != 0.0) reason = OSR_KSP_DIVERGED_BREAKDOWN;
+    else if (its >= sc[SC_MAXIT]) reason = OSR_KSP_DIVERGED_ITS;
+    if (reason != 0.0) {
+        sc[SC_REASON] = reason;
+        sc[SC_DONE] = 1.0;
+    }
 }
 
 }  // namespace osr
@@ -277,20 +325,24 @@ int ksp_setup_bs(osr_ksp* k, hipStream_t st) {
     return (int)k->host_sc[osr::SC_NSING];
 }
 
+// sc: the device stopping state the kernels test first, or null
 template <int BS>
-void spmv(osr_ksp* k, const double* x, double* y, hipStream_t st) {
+void spmv(osr_ksp* k, const double* x, double* y, const double* sc, hipStream_t st) {
     const unsigned g = (unsigned)((k->n + 255) / 256);
-    hipLaunchKernelGGL(osr::k_bsr_spmv<BS>, dim3(g), dim3(256), 0, st, k->nbr, k->browptr, k->bcols, k->bvals, x, y);
+    hipLaunchKernelGGL(osr::k_bsr_spmv<BS>, dim3(g), dim3(256), 0, st, k->nbr, k->browptr, k->bcols, k->bvals, x, y,
+                       sc);
 }
 
 template <int BS>
 void apply(osr_ksp* k, const double* y, double* z, hipStream_t st) {
     const unsigned g = (unsigned)((k->n + 255) / 256);
-    hipLaunchKernelGGL(osr::k_bj_apply<BS>, dim3(g), dim3(256), 0, st, k->nbr, (const double*)k->minv, y, z);
+    hipLaunchKernelGGL(osr::k_bj_apply<BS>, dim3(g), dim3(256), 0, st, k->nbr, (const double*)k->minv, y, z,
+                       (const double*)k->sc);
 }
 
 void dot(osr_ksp* k, const double* a, const double* b, const double* c, const double* d, int op, hipStream_t st) {
-    hipLaunchKernelGGL(osr::k_dot2, dim3(osr::DOT_BLOCKS), dim3(256), 0, st, k->n, a, b, c, d, k->part);
+    hipLaunchKernelGGL(osr::k_dot2, dim3(osr::DOT_BLOCKS), dim3(256), 0, st, k->n, a, b, c, d, k->part,
+                       (const double*)k->sc);
     hipLaunchKernelGGL(osr::k_dot_final, dim3(1), dim3(256), 0, st, (const double*)k->part, k->sc, op);
 }
 
@@ -314,11 +366,13 @@ int ksp_solve_bs(osr_ksp* k, const double* b, double* x, const osr_ksp_opts* o, 
     double* sh = ph + n;
     const unsigned g = (unsigned)((n + 255) / 256);
     if (!o->x0_nonzero) hipLaunchKernelGGL(osr::k_vec_zero, dim3(g), dim3(256), 0, st, n, x);
+    // a fresh solve: the stopping state cleared before the first dot
+    hipLaunchKernelGGL(osr::k_ksp_params, dim3(1), dim3(1), 0, st, k->sc, 0.0, 0.0, 0.0, 0.0);
     // |b| and r0 = b - A x0
     dot(k, b, b, nullptr, nullptr, osr::OP_RNORM, st);
     OSR_HIP_RET(read_sc(k, st));
     const double bnorm = std::sqrt(k->host_sc[osr::SC_RN2]);
-    spmv<BS>(k, x, t, st);
+    spmv<BS>(k, x, t, nullptr, st);
     hipLaunchKernelGGL(osr::k_vec_init, dim3(g), dim3(256), 0, st, n, b, (const double*)t, r, rhat, p, v);
     dot(k, r, r, nullptr, nullptr, osr::OP_INIT, st);
     OSR_HIP_RET(read_sc(k, st));
@@ -337,43 +391,39 @@ int ksp_solve_bs(osr_ksp* k, const double* b, double* x, const osr_ksp_opts* o, 
         return OSR_OK;
     }
     res->reason = OSR_KSP_DIVERGED_ITS;
-    for (int it = 1; it <= o->maxit; it++) {
-        dot(k, rhat, r, nullptr, nullptr, osr::OP_RHO, st);
-        hipLaunchKernelGGL(osr::k_vec_p, dim3(g), dim3(256), 0, st, n, (const double*)k->sc, (const double*)r,
-                           (const double*)v, p);
-        apply<BS>(k, p, ph, st);
-        spmv<BS>(k, ph, v, st);
-        dot(k, rhat, v, nullptr, nullptr, osr::OP_ALPHA, st);
-        hipLaunchKernelGGL(osr::k_vec_s, dim3(g), dim3(256), 0, st, n, (const double*)k->sc, (const double*)r,
-                           (const double*)v, s);
-        apply<BS>(k, s, sh, st);
-        spmv<BS>(k, sh, t, st);
-        dot(k, t, s, t, t, osr::OP_OMEGA, st);
-        hipLaunchKernelGGL(osr::k_vec_xr, dim3(g), dim3(256), 0, st, n, (const double*)k->sc, (const double*)ph,
-                           (const double*)sh, (const double*)s, (const double*)t, x, r);
-        dot(k, r, r, nullptr, nullptr, osr::OP_RNORM, st);
+    if (o->maxit < 1) return OSR_OK;
+    hipLaunchKernelGGL(osr::k_ksp_params, dim3(1), dim3(1), 0, st, k->sc, tol, o->atol, o->dtol * r0,
+                       (double)o->maxit);
+    const double* scd = k->sc;
+    int launched = 0, chunk = 1;
+    while (launched < o->maxit) {
+        const int m = std::min(chunk, o->maxit - launched);
+        for (int it = 0; it < m; it++) {
+            dot(k, rhat, r, nullptr, nullptr, osr::OP_RHO, st);
+            hipLaunchKernelGGL(osr::k_vec_p, dim3(g), dim3(256), 0, st, n, scd, (const double*)r, (const double*)v,
+                               p);
+            apply<BS>(k, p, ph, st);
+            spmv<BS>(k, ph, v, scd, st);
+            dot(k, rhat, v, nullptr, nullptr, osr::OP_ALPHA, st);
+            hipLaunchKernelGGL(osr::k_vec_s, dim3(g), dim3(256), 0, st, n, scd, (const double*)r, (const double*)v,
+                               s);
+            apply<BS>(k, s, sh, st);
+            spmv<BS>(k, sh, t, scd, st);
+            dot(k, t, s, t, t, osr::OP_OMEGA, st);
+            hipLaunchKernelGGL(osr::k_vec_xr, dim3(g), dim3(256), 0, st, n, scd, (const double*)ph,
+                               (const double*)sh, (const double*)s, (const double*)t, x, r);
+            dot(k, r, r, nullptr, nullptr, osr::OP_RNORM, st);
+            hipLaunchKernelGGL(osr::k_ksp_check, dim3(1), dim3(1), 0, st, k->sc);
+        }
         OSR_HIP(hipGetLastError());
+        launched += m;
+        chunk = std::min(2 * chunk, 16);
         OSR_HIP_RET(read_sc(k, st));
-        const double rn = std::sqrt(k->host_sc[osr::SC_RN2]);
-        res->iterations = it;
-        res->rnorm = rn;
-        if (!std::isfinite(rn)) {
-            res->reason = OSR_KSP_DIVERGED_NANORINF;
-            break;
-        }
-        if (rn <= tol) {
-            res->reason = rn <= o->atol ? OSR_KSP_CONVERGED_ATOL : OSR_KSP_CONVERGED_RTOL;
-            break;
-        }
-        if (rn >= o->dtol * r0) {
-            res->reason = OSR_KSP_DIVERGED_DTOL;
-            break;
-        }
-        if (k->host_sc[osr::SC_FLAG] != 0.0) {
-            res->reason = OSR_KSP_DIVERGED_BREAKDOWN;
-            break;
-        }
+        if (k->host_sc[osr::SC_DONE] != 0.0) break;
     }
+    res->iterations = (int)k->host_sc[osr::SC_ITS];
+    res->rnorm = k->host_sc[osr::SC_RN];
+    res->reason = (int)k->host_sc[osr::SC_REASON];
     return OSR_OK;
 }
 

@@ -194,3 +194,34 @@ def test_reasons(gpu):
     ksp = gpu.Ksp(bs, nbr)
     assert ksp.setup(dev(browptr), dev(cols), dev(blocks)) == 1
     ksp.close()
+
+
+def test_device_stopping_test(gpu):
+    """The stopping test runs on the device and the host launches iterations
+    in chunks: a solve that converges at iteration m leaves bitwise the
+    iterate of a solve capped at m iterations, however many iterations past
+    m were launched (they are empty), and the reported norm is |b - A x| of
+    that iterate."""
+    nbr, bs = 400, 4
+    g = np.random.Generator(np.random.PCG64(5))
+    browptr, cols, vals = _block_tridiag(nbr, bs, 5, 4.0)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    ksp = gpu.Ksp(bs, nbr)
+    assert ksp.setup(dev(browptr), dev(cols), dev(vals)) == 0
+    b = dev(g.normal(size=nbr * bs))
+    x_conv, info = ksp.solve(b, rtol=1e-10, maxit=1000)
+    assert info["reason"] == "converged_rtol", info
+    m = info["iterations"]
+    assert m > 3, info
+    x_cap, info_cap = ksp.solve(b, rtol=1e-300, maxit=m)
+    assert info_cap["reason"] == "diverged_its" and info_cap["iterations"] == m
+    assert torch.equal(x_conv, x_cap)
+    assert info_cap["rnorm"] == info["rnorm"]
+    A = sp.bsr_matrix((vals, cols.astype(np.int64), browptr), shape=(nbr * bs, nbr * bs))
+    r = b.cpu().numpy() - A @ x_conv.cpu().numpy()
+    assert abs(np.linalg.norm(r) - info["rnorm"]) <= 1e-6 * np.linalg.norm(b.cpu().numpy())
+    # each cap from 1 to m stops exactly there
+    for cap in (1, 2, 3, m - 1):
+        _, inf = ksp.solve(b, rtol=1e-300, maxit=cap)
+        assert inf["iterations"] == cap and inf["reason"] == "diverged_its", (cap, inf)
+    ksp.close()
