@@ -1872,7 +1872,7 @@ __global__ __launch_bounds__(256) void k_red_local(AsmArgs A, const double* Wl, 
 // quantities by cross-lane reads, lane 31 the residual column, and the group
 // eliminates as k_red_local does.
 template <int NC>
-__global__ __launch_bounds__(256, 4) void k_red_locel(AsmArgs A, const double* F, double* S, double* t) {
+__global__ __launch_bounds__(256, 3) void k_red_locel(AsmArgs A, const double* F, double* S, double* t) {
     constexpr int NLM = NC + 3, NP = NC + 1;
     static_assert(NC <= 16, "one component per lane of a 16-lane half");
     static_assert(2 * NC + 4 < RED_GROUP, "a block's unknowns and its residual column fit one lane group");
